@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""Benchmark: Chebyshev-filter HBM throughput on one MI355X.
+
+Workload: the operator-level filter case -- 96^3 periodic grid, h = 0.3 Bohr,
+order-12 stencil, 512 vectors uniform[-0.5, 0.5] from a seed, Veff = 16 seeded
+Gaussians, 64 projector sites x 8 projectors (2 radial x l<=1, cutoff 6 Bohr),
+degree 24, bounds from a 10-step Lanczos.  A step is one full degree-24 filter
+of the 512-vector block (inputs resident in HBM); consecutive steps filter the
+previous output (the scaled recurrence keeps magnitudes bounded).
+
+value = algorithmic bytes per filter / measured filter time, in GB/s, for the
+FP32 (mixed-precision) filter; the FP64 filter is timed beside it.
+Algorithmic bytes per filter:
+    s*Nd*Ns*(2 + 3(m-1)) + m*s*Nd + 2*m*s*sum(Chi)        (s = 4 or 8)
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "SCF-iteration time & filter HBM GB/s, mixed vs FP64, at ΔE≤1e-5 Ha/atom"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(nd, ns, m, s, chi_elems):
+    return s * nd * ns * (2 + 3 * (m - 1)) + m * s * nd + 2 * m * s * chi_elems
+
+
+def stencil_bytes(nd, ns, m, s):
+    """Bytes the fused stencil/recurrence launches must move over one filter:
+    first launch reads X and writes Y, the m-1 others read Y_k, Y_{k-1} and
+    write Y_{k+1}; every launch reads Veff once per column block."""
+    return s * nd * ns * (2 + 3 * (m - 1)) + m * s * nd
+
+
+def bounds_lanczos(ctx, wl):
+    from sparc_amd.workloads import uniform_block
+    x0 = uniform_block(wl.grid.nd, 1, wl.lanczos_seed, 0.0, 1.0)[:, 0]
+    emin, emax, _ = ctx.Lanczos(x0, 0.0, 0.0, 10)
+    emax *= 1.01
+    emin -= 0.1
+    return emin, emax, 0.5 * (emin + emax)
+
+
+def time_filter(ctx, gpu, wl, dtype, steps, warmup, bounds):
+    a0, b, a = bounds
+    X = ctx.block(wl.ncol, dtype).fill_uniform(wl.psi_seed, -0.5, 0.5)
+    Y = ctx.block(wl.ncol, dtype)
+    for _ in range(warmup):
+        ctx.chebyshev_filter_dev(X, Y, wl.degree, a, b, a0)
+        X, Y = Y, X
+    ctx.sync()
+    ctx.timer_reset()
+    ctx.timer_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.chebyshev_filter_dev(X, Y, wl.degree, a, b, a0)
+        X, Y = Y, X
+    ctx.sync()
+    t1 = time.perf_counter()
+    ctx.timer_enable(False)
+    out = {"wall_ms": (t1 - t0) * 1e3 / max(steps, 1)}
+    for name in ("filter", "stencil", "nonlocal"):
+        tot, cnt = ctx.timer(name)
+        out[name] = (tot, cnt)
+    finite = bool(np.isfinite(X.download(1)).all())
+    X.free()
+    Y.free()
+    out["finite"] = finite
+    return out
+
+
+def cpu_baseline(wl, ncol_sample=48, degree=None):
+    """FP64 oracle filter on the host cores over a bounded sample of the same
+    workload (same grid, Veff, projectors; ncol_sample vectors)."""
+    from oracle import oracle as O
+    from sparc_amd.workloads import uniform_block
+    threads = min(16, os.cpu_count() or 1)
+    O.set_threads(threads)
+    o = O.Oracle(wl.grid, wl.veff, wl.nloc)
+    m = degree or wl.degree
+    x0 = uniform_block(wl.grid.nd, 1, wl.lanczos_seed, 0.0, 1.0)[:, 0]
+    emin, emax, _ = o.lanczos(x0, 0.0, 0.0, 10)
+    emax *= 1.01
+    emin -= 0.1
+    a = 0.5 * (emin + emax)
+    X = uniform_block(wl.grid.nd, ncol_sample, wl.psi_seed, -0.5, 0.5)
+    t0 = time.perf_counter()
+    o.chebyshev_filter(X, m, a, emax, emin)
+    dt = time.perf_counter() - t0
+    chi = int(wl.nloc.chi.size)
+    gbs = algorithmic_bytes(wl.grid.nd, ncol_sample, m, 8, chi) / dt / 1e9
+    return {"value": round(gbs, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle filter, same grid/Veff/projectors, {ncol_sample} of {wl.ncol} "
+                      f"vectors, degree {m}, {dt:.2f} s"}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=96)
+    ap.add_argument("--ncol", type=int, default=512)
+    ap.add_argument("--degree", type=int, default=24)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is described in DESIGN.md, not run)"}))
+        return 2
+
+    from sparc_amd import gpu
+    from sparc_amd.workloads import operator_filter_workload
+
+    t_setup = time.perf_counter()
+    wl = operator_filter_workload(n=args.n, ncol=args.ncol, degree=args.degree)
+    nd, ns, m = wl.grid.nd, wl.ncol, wl.degree
+    chi = int(wl.nloc.chi.size)
+    print(f"[bench] workload {args.n}^3 x {ns}, degree {m}, {wl.nloc.n_influence} sites, "
+          f"chi {chi} values, setup {time.perf_counter() - t_setup:.1f}s", flush=True)
+    ctx = gpu.Context(wl.grid)
+    ctx.set_veff(wl.veff)
+    ctx.set_nonlocal(wl.nloc)
+    bounds = bounds_lanczos(ctx, wl)
+    print(f"[bench] bounds eigmin {bounds[0]:.6f} eigmax {bounds[1]:.6f} cutoff {bounds[2]:.6f}",
+          flush=True)
+
+    r32 = time_filter(ctx, gpu, wl, gpu.F32, args.steps, args.warmup, bounds)
+    ms32 = r32["filter"][0] / max(r32["filter"][1], 1)
+    gbs32 = algorithmic_bytes(nd, ns, m, 4, chi) / (ms32 * 1e-3) / 1e9
+    st_ms = r32["stencil"][0] / max(r32["stencil"][1], 1)
+    st_gbs = stencil_bytes(nd, ns, m, 4) / m / (st_ms * 1e-3) / 1e9
+    print(f"[bench] fp32 filter {ms32:.2f} ms ({gbs32:.0f} GB/s), stencil launch {st_ms:.3f} ms "
+          f"({st_gbs:.0f} GB/s), nonlocal {r32['nonlocal'][0] / max(r32['nonlocal'][1], 1):.3f} ms",
+          flush=True)
+    res = {
+        "metric": METRIC,
+        "value": round(gbs32, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r32["wall_ms"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"Chebyshev filter operator case: {args.n}^3 periodic grid h=0.3, "
+                               f"order 12, {ns} vectors, degree {m}, 16-Gaussian Veff, "
+                               f"{wl.nloc.n_influence} sites x 8 projectors rc 6, bounds from "
+                               f"10-step Lanczos",
+                   "nd": nd, "ncol": ns, "degree": m},
+        "filter_ms": {"f32": round(ms32, 3)},
+        "filter_gbs": {"f32": round(gbs32, 2)},
+        "roofline": {"bound": "hbm", "achieved": round(st_gbs, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(st_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "stencil_tiled<float,32,16,2>",
+                     "per_launch_algorithmic_bytes": stencil_bytes(nd, ns, m, 4) / m},
+        "finite": r32["finite"],
+    }
+    if not args.no_fp64:
+        r64 = time_filter(ctx, gpu, wl, gpu.F64, args.steps, args.warmup, bounds)
+        ms64 = r64["filter"][0] / max(r64["filter"][1], 1)
+        gbs64 = algorithmic_bytes(nd, ns, m, 8, chi) / (ms64 * 1e-3) / 1e9
+        res["filter_ms"]["f64"] = round(ms64, 3)
+        res["filter_gbs"]["f64"] = round(gbs64, 2)
+        res["speedup_f32_vs_f64"] = round(ms64 / ms32, 3)
+        print(f"[bench] fp64 filter {ms64:.2f} ms ({gbs64:.0f} GB/s)", flush=True)
+    ctx.close()
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline ...", flush=True)
+        res["cpu_baseline"] = cpu_baseline(wl)
+    print(json.dumps(res))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

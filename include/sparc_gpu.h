@@ -1,0 +1,191 @@
+/*
+ * sparc_gpu.h -- C ABI of libsparcgpu.so, the MI355X (gfx950) implementation
+ * of SPARC's Chebyshev-filtered subspace iteration (CheFSI) and AAR
+ * electrostatics hot path.
+ *
+ * One context owns one orthogonal grid on one GPU (the "one MPI rank owns the
+ * whole domain" case of SPARC).  Operator data (Veff, nonlocal projectors) is
+ * uploaded once and stays in HBM.  Two families of entry points exist:
+ *
+ *  - Host-pointer drop-ins with the argument meaning of the reference C entry
+ *    points (column-major blocks, leading dimensions ldi/ldo, FP64 host data).
+ *    Each one cites the reference function it replaces.  They copy their
+ *    inputs to the device, run, and copy results back.
+ *  - Device-resident variants (`*_dev`) working on sgpu_block handles, so
+ *    that Psi stays in HBM across filter -> projection -> rotation.
+ *
+ * Every function returns an int status: SGPU_OK (0) on success, a negative
+ * SGPU_ERR_* on failure (message via sgpu_last_error()), or SGPU_NOT_CONVERGED
+ * (1) where an iterative method stopped at its iteration limit.  The reference
+ * functions are void and exit() on failure; the status is this library's
+ * addition (see each entry).
+ *
+ * Precision: SGPU_F64 runs everything in FP64.  SGPU_F32 stores the vector
+ * block in FP32 and runs the stencil/Veff/Chebyshev/nonlocal arithmetic in
+ * FP32; projections return FP64 matrices (FP32 products, FP64 accumulation
+ * across tiles), and the dense eigensolve is always FP64.
+ */
+#ifndef SPARC_GPU_H
+#define SPARC_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SGPU_OK              0
+#define SGPU_NOT_CONVERGED   1
+#define SGPU_ERR_ARG        -1
+#define SGPU_ERR_HIP        -2
+#define SGPU_ERR_NOMEM      -3
+#define SGPU_ERR_STATE      -4
+#define SGPU_ERR_SOLVER     -5
+
+#define SGPU_F64 0
+#define SGPU_F32 1
+
+typedef struct sgpu_context sgpu_context;
+typedef struct sgpu_block sgpu_block;
+
+/* ---- context -------------------------------------------------------- */
+
+/* Create a context for an nx*ny*nz orthogonal grid with mesh (hx,hy,hz),
+ * boundary conditions bc* (0 periodic, 1 Dirichlet; pSPARC->BCx convention,
+ * src/lapVecRoutines.c:330-332) and FD order (even, <= 16).  Computes the
+ * second-derivative weights of src/initialization.c:2113-2147. */
+int sgpu_create(sgpu_context **ctx, int device, int nx, int ny, int nz,
+                double hx, double hy, double hz, int bcx, int bcy, int bcz, int order);
+/* Destroying a context releases the storage of its remaining blocks; their
+ * handles stay valid only for sgpu_block_destroy. */
+int sgpu_destroy(sgpu_context *ctx);
+const char *sgpu_last_error(void);
+/* Scaled D2 stencil coefficients D2_stencil_coeffs_{x,y,z}[0..order/2]. */
+int sgpu_get_stencil_coeffs(const sgpu_context *ctx, double *d2x, double *d2y, double *d2z);
+/* Synchronise the context's stream. */
+int sgpu_sync(sgpu_context *ctx);
+
+/* Veff_loc_dmcomm (one spin channel, Nd values).  Replaces the field read by
+ * Hamiltonian_vectors_mult (src/hamiltonianVecRoutines.c:45-49). */
+int sgpu_set_veff(sgpu_context *ctx, const double *veff);
+
+/* Nonlocal projectors, flattened from ATOM_NLOC_INFLUENCE_OBJ / NLOC_PROJ_OBJ
+ * (src/include/isddft.h:202-216,247-265): influence entry a has ndc[a] grid
+ * points grid_pos[pos_off[a]...], Chi as a column-major ndc[a] x nproj[a]
+ * block at chi[chi_off[a]], and projector rows starting at alpha_off[a]
+ * (IP_displ of its parent atom; images of one atom share rows).  gamma[r] is
+ * Gamma_{l,p} of projector row r (n_alpha rows).  dV is the volume element. */
+int sgpu_set_nonlocal(sgpu_context *ctx, int n_influence, const int *ndc, const int *nproj,
+                      const long *pos_off, const int *grid_pos, const long *chi_off,
+                      const double *chi, const int *alpha_off, int n_alpha,
+                      const double *gamma, double dV);
+
+/* ---- device blocks -------------------------------------------------- */
+
+/* A block of ncol grid vectors in HBM, column-major, dtype SGPU_F64/F32. */
+int sgpu_block_create(sgpu_context *ctx, int ncol, int dtype, sgpu_block **blk);
+int sgpu_block_destroy(sgpu_block *blk);
+int sgpu_block_ncol(const sgpu_block *blk);
+int sgpu_block_dtype(const sgpu_block *blk);
+long sgpu_block_ld(const sgpu_block *blk);
+/* Copy ncol host columns (leading dimension ld) in / out, converting type. */
+int sgpu_block_upload(sgpu_block *blk, const double *host, long ld, int ncol);
+int sgpu_block_download(const sgpu_block *blk, double *host, long ld, int ncol);
+/* Fill with uniform [lo,hi) values from a counter-based hash of (seed, index). */
+int sgpu_block_fill_uniform(sgpu_block *blk, unsigned long long seed, double lo, double hi);
+/* Copy one block into another of the same shape (dtype may differ). */
+int sgpu_block_copy(const sgpu_block *src, sgpu_block *dst);
+
+/* ---- reference entry points: host-pointer drop-ins ------------------- */
+
+/* Lapx = (Lap + c I) x.  Replaces Lap_vec_mult (src/lapVecRoutines.c:37-58). */
+int sgpu_Lap_vec_mult(sgpu_context *ctx, int ncol, double c, const double *x, int ldi,
+                      double *Lapx, int ldo);
+/* y = (a Lap + b diag(v) + c I) x.  Replaces Lap_plus_diag_vec_mult_orth
+ * (src/lapVecRoutines.c:306-611); v == NULL or |b| < 1e-14 drops the diagonal. */
+int sgpu_Lap_plus_diag_vec_mult(sgpu_context *ctx, int ncol, double a, double b, double c,
+                                const double *v, const double *x, int ldi, double *y, int ldo);
+/* Hx += Vnl x.  Replaces Vnl_vec_mult (src/nlocVecRoutines.c:798-883). */
+int sgpu_Vnl_vec_mult(sgpu_context *ctx, int ncol, const double *x, int ldi, double *Hx, int ldo);
+/* Hx = (-1/2 Lap + Veff + c) x + Vnl x.  Replaces Hamiltonian_vectors_mult
+ * (src/hamiltonianVecRoutines.c:45-121); Hx is overwritten. */
+int sgpu_Hamiltonian_vectors_mult(sgpu_context *ctx, int ncol, double c, const double *x, int ldi,
+                                  double *Hx, int ldo);
+/* Chebyshev filter of degree m with cutoff a, upper bound b, lower bound a0.
+ * Replaces ChebyshevFiltering (src/eigenSolver.c:722-798): Y receives the
+ * filtered block and X is clobbered (ends as Y_{m-1}).  dtype selects the
+ * arithmetic (SGPU_F64 or SGPU_F32). */
+int sgpu_ChebyshevFiltering(sgpu_context *ctx, double *X, int ldi, double *Y, int ldo, int ncol,
+                            int m, double a, double b, double a0, int dtype);
+/* HY = H Y, Mp = Y'Y, Hp = Y'HY (Ns x Ns, column-major).  Replaces
+ * DP_Project_Hamiltonian (src/eigenSolver.c:939-1082) on one rank. */
+int sgpu_DP_Project_Hamiltonian(sgpu_context *ctx, const double *Y, int ldi, double *HY, int ldo,
+                                int ncol, double *Hp, double *Mp, int dtype);
+/* Solve Hp Q = Mp Q Lambda (itype 1, upper triangles read).  Replaces the
+ * LAPACKE_dsygvd call of DP_Solve_Generalized_EigenProblem
+ * (src/eigenSolver.c:1262-1375); Hp is overwritten by Q, lambda ascending.
+ * Runs rocSOLVER dsygvd in FP64 on the device.  Returns SGPU_ERR_SOLVER with
+ * the LAPACK info in the message when the factorisation fails (the reference
+ * ignores info). */
+int sgpu_DP_Solve_Generalized_EigenProblem(sgpu_context *ctx, int n, double *Hp, double *Mp,
+                                           double *lambda);
+/* Psi_rot = Y Q.  Replaces DP_Subspace_Rotation (src/eigenSolver.c:1386-1439). */
+int sgpu_DP_Subspace_Rotation(sgpu_context *ctx, const double *Y, int ldy, const double *Q,
+                              int ns, double *Psi_rot, int ldo, int dtype);
+
+/* Residual / preconditioner callbacks of AAR with the argument meaning of
+ * the reference's res_fun / precond_fun (src/linearSolver.c:37-43), acting on
+ * FP64 device vectors of length N on the context's stream. */
+typedef void (*sgpu_res_fn)(void *user, int N, double c, const double *x_dev,
+                            const double *b_dev, double *r_dev);
+typedef void (*sgpu_precond_fn)(void *user, int N, double c, const double *r_dev,
+                                double *f_dev);
+/* Alternating Anderson-Richardson.  Replaces AAR (src/linearSolver.c:37-159).
+ * res == NULL selects the built-in poisson_residual (src/lapVecRoutines.c:69)
+ * and precond == NULL the built-in Jacobi_preconditioner
+ * (src/electrostatics.c:1682).  x and b are host arrays of length N = Nd.
+ * iters / relres (may be NULL) receive the iteration count and ||r||/||b||
+ * at the last norm evaluation.  Returns SGPU_NOT_CONVERGED if max_iter was
+ * reached (the reference returns silently). */
+int sgpu_AAR(sgpu_context *ctx, sgpu_res_fn res, sgpu_precond_fn precond, void *user, double c,
+             int N, double *x, const double *b, double omega, double beta, int m, int p,
+             double tol, int max_iter, int *iters, double *relres);
+
+/* ---- device-resident variants ---------------------------------------- */
+
+/* y = (a Lap + b diag(Veff) + c) x on blocks (diagonal from sgpu_set_veff). */
+int sgpu_lap_plus_diag_dev(sgpu_context *ctx, double a, double b, double c,
+                           const sgpu_block *x, sgpu_block *y);
+/* Hx = (H + c) x, Hx overwritten. */
+int sgpu_hamiltonian_dev(sgpu_context *ctx, double c, const sgpu_block *x, sgpu_block *Hx);
+/* Chebyshev filter on blocks; same contract as sgpu_ChebyshevFiltering
+ * (Y result, X = Y_{m-1}); the two handles may exchange their storage. */
+int sgpu_chebyshev_filter_dev(sgpu_context *ctx, sgpu_block *X, sgpu_block *Y, int m,
+                              double a, double b, double a0);
+/* HY = H Y and Mp = Y'Y, Hp = Y'HY into host arrays (FP64). */
+int sgpu_project_dev(sgpu_context *ctx, const sgpu_block *Y, sgpu_block *HY, double *Hp,
+                     double *Mp);
+/* out = Y Q with host Q (ns x ns FP64). */
+int sgpu_rotate_dev(sgpu_context *ctx, const sgpu_block *Y, const double *Q, sgpu_block *out);
+/* Lanczos bounds of H (src/eigenSolver.c:1920-2129) from host start vector
+ * x0 (FP64); returns the iteration count in *iters. */
+int sgpu_lanczos(sgpu_context *ctx, const double *x0, double tol_min, double tol_max, int maxit,
+                 double *eigmin, double *eigmax, int *iters);
+/* One CheFSI pass (src/eigenSolver.c:269-445) on resident blocks: filter X
+ * into Y with bounds (cutoff a, eigmax b, eigmin a0), project, solve the
+ * FP64 generalized problem, rotate into X.  lambda receives Ns eigenvalues. */
+int sgpu_chefsi_dev(sgpu_context *ctx, sgpu_block *X, sgpu_block *Y, int m, double a, double b,
+                    double a0, double *lambda);
+
+/* ---- timing ---------------------------------------------------------- */
+
+/* Average device time (ms) of the last timed launches of the named kernel
+ * group ("filter_stencil", "nonlocal", "filter", ...), measured with HIP
+ * events on the context's stream; *count gets the number of samples. */
+int sgpu_timer_get(sgpu_context *ctx, const char *name, double *total_ms, int *count);
+int sgpu_timer_reset(sgpu_context *ctx);
+/* Enable (1) or disable (0) per-kernel event timing (default off). */
+int sgpu_timer_enable(sgpu_context *ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

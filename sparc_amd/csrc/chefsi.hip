@@ -1,0 +1,565 @@
+// Hamiltonian apply, Chebyshev filter, Rayleigh-Ritz projection, FP64
+// generalized eigensolve, subspace rotation, Lanczos bounds and one CheFSI
+// pass, plus the host-pointer drop-ins of the reference entry points.
+#include <rocsolver/rocsolver.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "sgpu_internal.h"
+
+namespace {
+
+int check_block(const sgpu_context *ctx, const sgpu_block *b, const char *what)
+{
+    if (!b || b->ctx != ctx) {
+        sgpu_set_error("%s: block missing or from another context", what);
+        return SGPU_ERR_ARG;
+    }
+    return SGPU_OK;
+}
+
+int check_pair(const sgpu_context *ctx, const sgpu_block *a, const sgpu_block *b, const char *what)
+{
+    SGPU_CHECK(check_block(ctx, a, what));
+    SGPU_CHECK(check_block(ctx, b, what));
+    if (a->ncol != b->ncol || a->dtype != b->dtype) {
+        sgpu_set_error("%s: blocks differ in shape or type", what);
+        return SGPU_ERR_ARG;
+    }
+    return SGPU_OK;
+}
+
+const void *veff_for(const sgpu_context *ctx, int dtype)
+{
+    return dtype == SGPU_F64 ? (const void *)ctx->veff64 : (const void *)ctx->veff32;
+}
+
+// out = s1 * (H + c) x + s2 * z   (z may alias out; x must not).
+int apply_h(sgpu_context *ctx, int dtype, int ncol, double c, const void *x, long ldx, void *out,
+            long ldo, const void *z, long ldz, double s1, double s2)
+{
+    if (!ctx->has_veff) {
+        sgpu_set_error("Hamiltonian apply before sgpu_set_veff");
+        return SGPU_ERR_STATE;
+    }
+    sgpu_tmark t;
+    SGPU_CHECK(sgpu_tbegin(ctx, &t));
+    SGPU_CHECK(sgpu_launch_stencil(ctx, dtype, ncol, -0.5, 1.0, c, veff_for(ctx, dtype), x, ldx, out,
+                                   ldo, z, ldz, s1, s2, true));
+    SGPU_CHECK(sgpu_tend(ctx, &t, "stencil"));
+    if (ctx->nl.n_infl > 0) {
+        SGPU_CHECK(sgpu_tbegin(ctx, &t));
+        SGPU_CHECK(sgpu_launch_vnl(ctx, dtype, ncol, x, ldx, out, ldo, s1));
+        SGPU_CHECK(sgpu_tend(ctx, &t, "nonlocal"));
+    }
+    return SGPU_OK;
+}
+
+// Chebyshev filter on raw device storage (ChebyshevFiltering,
+// src/eigenSolver.c:722-798).  On return *res points at the buffer holding
+// Y_m and *prev at the one holding Y_{m-1}; bx starts as X, by as Y.
+int filter_raw(sgpu_context *ctx, int dtype, int ncol, void *bx, void *by, long ld, int m,
+               double a, double b, double a0, void **res, void **prev)
+{
+    const double e = 0.5 * (b - a), c = 0.5 * (b + a);
+    double sigma = e / (a0 - c);
+    const double sigma1 = sigma, gamma = 2.0 / sigma1;
+    sgpu_tmark tall;
+    SGPU_CHECK(sgpu_tbegin(ctx, &tall));
+    // Y = (sigma1 / e) (H - c) X
+    SGPU_CHECK(apply_h(ctx, dtype, ncol, -c, bx, ld, by, ld, nullptr, 0, sigma1 / e, 0.0));
+    void *P = by, *Q = bx;
+    for (int j = 1; j < m; j++) {
+        const double sigma2 = 1.0 / (gamma - sigma);
+        const double vs = 2.0 * sigma2 / e, vs2 = sigma * sigma2;
+        // Q <- vs (H - c) P - vs2 Q   (in place over Y_{j-1})
+        SGPU_CHECK(apply_h(ctx, dtype, ncol, -c, P, ld, Q, ld, Q, ld, vs, -vs2));
+        std::swap(P, Q);
+        sigma = sigma2;
+    }
+    SGPU_CHECK(sgpu_tend(ctx, &tall, "filter"));
+    *res = P;
+    *prev = Q;
+    return SGPU_OK;
+}
+
+// Sum a [nslice][n*n] FP32 partial stack into an FP64 n*n matrix.
+__global__ void sum_partials(const float *__restrict__ part, int nslice, long nn,
+                             double *__restrict__ out)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nn) return;
+    double s = 0.0;
+    for (int k = 0; k < nslice; k++) s += (double)part[(long)k * nn + t];
+    out[t] = s;
+}
+
+// C (n x n, FP64, device) = A' B over nd rows; split-K in FP32 for SGPU_F32.
+int gram_tn(sgpu_context *ctx, int dtype, int n, long nd, const void *A, long lda, const void *B,
+            long ldb, double *C)
+{
+    if (dtype == SGPU_F64) {
+        const double one = 1.0, zero = 0.0;
+        SGPU_BLAS(rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host));
+        SGPU_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, n, n,
+                                (rocblas_int)nd, &one, (const double *)A, (rocblas_int)lda,
+                                (const double *)B, (rocblas_int)ldb, &zero, C, n));
+        return SGPU_OK;
+    }
+    // FP32 products with FP32 accumulation inside K-slices of kslice rows and
+    // FP64 accumulation across slices
+    const long kslice = 8192;
+    const int nslice = (int)((nd + kslice - 1) / kslice);
+    const long nn = (long)n * n;
+    void *buf = nullptr;
+    SGPU_CHECK(sgpu_scratch(ctx, (size_t)nslice * nn * sizeof(float), &buf));
+    float *part = (float *)buf;
+    const float one = 1.f, zero = 0.f;
+    SGPU_BLAS(rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host));
+    const long full = nd / kslice;
+    if (full > 0)
+        SGPU_BLAS(rocblas_sgemm_strided_batched(
+            ctx->blas, rocblas_operation_transpose, rocblas_operation_none, n, n,
+            (rocblas_int)kslice, &one, (const float *)A, (rocblas_int)lda, kslice,
+            (const float *)B, (rocblas_int)ldb, kslice, &zero, part, n, nn, (rocblas_int)full));
+    const long rem = nd - full * kslice;
+    if (rem > 0)
+        SGPU_BLAS(rocblas_sgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, n, n,
+                                (rocblas_int)rem, &one, (const float *)A + full * kslice,
+                                (rocblas_int)lda, (const float *)B + full * kslice,
+                                (rocblas_int)ldb, &zero, part + full * nn, n));
+    sum_partials<<<(unsigned)((nn + 255) / 256), 256, 0, ctx->stream>>>(part, nslice, nn, C);
+    SGPU_HIP(hipGetLastError());
+    return SGPU_OK;
+}
+
+int project_raw(sgpu_context *ctx, int dtype, int ncol, const void *Y, void *HY, long ld,
+                double *Hp_host, double *Mp_host)
+{
+    sgpu_tmark t;
+    SGPU_CHECK(apply_h(ctx, dtype, ncol, 0.0, Y, ld, HY, ld, nullptr, 0, 1.0, 0.0));
+    double *dmat = nullptr;
+    const long nn = (long)ncol * ncol;
+    SGPU_CHECK(sgpu_reduce_buf(ctx, (size_t)(2 * nn), &dmat));
+    SGPU_CHECK(sgpu_tbegin(ctx, &t));
+    SGPU_CHECK(gram_tn(ctx, dtype, ncol, ctx->nd, Y, ld, Y, ld, dmat));
+    SGPU_CHECK(gram_tn(ctx, dtype, ncol, ctx->nd, Y, ld, HY, ld, dmat + nn));
+    SGPU_CHECK(sgpu_tend(ctx, &t, "project_gemm"));
+    SGPU_HIP(hipMemcpyAsync(Mp_host, dmat, nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    SGPU_HIP(hipMemcpyAsync(Hp_host, dmat + nn, nn * sizeof(double), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    SGPU_HIP(hipStreamSynchronize(ctx->stream));
+    return SGPU_OK;
+}
+
+int rotate_raw(sgpu_context *ctx, int dtype, int ns, const void *Y, long ldy, const double *Q_host,
+               void *out, long ldo)
+{
+    sgpu_tmark t;
+    const long nn = (long)ns * ns;
+    void *qd = nullptr;
+    double *q64 = nullptr;
+    SGPU_CHECK(sgpu_reduce_buf(ctx, (size_t)nn, &q64));
+    SGPU_HIP(hipMemcpyAsync(q64, Q_host, nn * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    SGPU_CHECK(sgpu_tbegin(ctx, &t));
+    SGPU_BLAS(rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host));
+    if (dtype == SGPU_F64) {
+        const double one = 1.0, zero = 0.0;
+        SGPU_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none,
+                                (rocblas_int)ctx->nd, ns, ns, &one, (const double *)Y,
+                                (rocblas_int)ldy, q64, ns, &zero, (double *)out, (rocblas_int)ldo));
+    } else {
+        SGPU_CHECK(sgpu_scratch(ctx, nn * sizeof(float), &qd));
+        SGPU_CHECK(sgpu_convert(ctx, q64, SGPU_F64, qd, SGPU_F32, nn));
+        const float one = 1.f, zero = 0.f;
+        SGPU_BLAS(rocblas_sgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none,
+                                (rocblas_int)ctx->nd, ns, ns, &one, (const float *)Y,
+                                (rocblas_int)ldy, (const float *)qd, ns, &zero, (float *)out,
+                                (rocblas_int)ldo));
+    }
+    SGPU_CHECK(sgpu_tend(ctx, &t, "rotate_gemm"));
+    SGPU_HIP(hipStreamSynchronize(ctx->stream));
+    return SGPU_OK;
+}
+
+// dsygvd on the device (FP64).  Hp (host, n x n) is overwritten by Q.
+int eigensolve(sgpu_context *ctx, int n, double *Hp, double *Mp, double *lambda)
+{
+    const long nn = (long)n * n;
+    void *buf = nullptr;
+    // A, B, D, E, info in one scratch allocation
+    const size_t bytes = (2 * nn + 2 * (size_t)n) * sizeof(double) + 64;
+    SGPU_CHECK(sgpu_scratch(ctx, bytes, &buf));
+    double *dA = (double *)buf, *dB = dA + nn, *dD = dB + nn, *dE = dD + n;
+    rocblas_int *dinfo = (rocblas_int *)(dE + n);
+    SGPU_HIP(hipMemcpyAsync(dA, Hp, nn * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    SGPU_HIP(hipMemcpyAsync(dB, Mp, nn * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    sgpu_tmark t;
+    SGPU_CHECK(sgpu_tbegin(ctx, &t));
+    SGPU_BLAS(rocsolver_dsygvd(ctx->blas, rocblas_eform_ax, rocblas_evect_original,
+                               rocblas_fill_upper, n, dA, n, dB, n, dD, dE, dinfo));
+    SGPU_CHECK(sgpu_tend(ctx, &t, "eigensolve"));
+    rocblas_int info = 0;
+    SGPU_HIP(hipMemcpyAsync(Hp, dA, nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    SGPU_HIP(hipMemcpyAsync(lambda, dD, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    SGPU_HIP(hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, ctx->stream));
+    SGPU_HIP(hipStreamSynchronize(ctx->stream));
+    if (info != 0) {
+        sgpu_set_error("rocsolver_dsygvd info = %d", (int)info);
+        return SGPU_ERR_SOLVER;
+    }
+    return SGPU_OK;
+}
+
+// ---- Lanczos vector kernels ----
+__global__ void lz_orth(double *__restrict__ vjp1, const double *__restrict__ vj,
+                        double *__restrict__ vjm1, double a, double b, long n)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    vjp1[i] -= a * vj[i] + b * vjm1[i];
+    vjm1[i] = vj[i];
+}
+
+__global__ void vscale(double *__restrict__ y, const double *__restrict__ x, double s, long n)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = s * x[i];
+}
+
+__global__ void vaxpy(double *__restrict__ y, const double *__restrict__ x, double s, long n)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += s * x[i];
+}
+
+int sturm_count(int n, const double *d, const double *e, double x)
+{
+    int cnt = 0;
+    double q = d[0] - x;
+    if (q < 0) cnt++;
+    for (int i = 1; i < n; i++) {
+        if (q == 0.0) q = 1e-300;
+        q = d[i] - x - e[i - 1] * e[i - 1] / q;
+        if (q < 0) cnt++;
+    }
+    return cnt;
+}
+
+// Extreme eigenvalues of the symmetric tridiagonal matrix (diagonal d,
+// off-diagonal e) by Sturm-sequence bisection; the reference takes the ends
+// of LAPACKE_dsterf's spectrum (src/eigenSolver.c:2072-2081).
+void tridiag_extremes(int n, const double *d, const double *e, double *lmin, double *lmax)
+{
+    double lo = d[0], hi = d[0];
+    for (int i = 0; i < n; i++) {
+        double r = (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i < n - 1 ? std::fabs(e[i]) : 0.0);
+        lo = std::min(lo, d[i] - r);
+        hi = std::max(hi, d[i] + r);
+    }
+    const double span = hi - lo + 1e-300;
+    for (int which = 0; which < 2; which++) {
+        double l = lo - 1e-12 * span, h = hi + 1e-12 * span;
+        const int target = which == 0 ? 1 : n;
+        for (int it = 0; it < 200 && h - l > 4e-16 * (std::fabs(l) + std::fabs(h)) + 1e-300; it++) {
+            double mid = 0.5 * (l + h);
+            if (sturm_count(n, d, e, mid) >= target) h = mid; else l = mid;
+        }
+        (which == 0 ? *lmin : *lmax) = 0.5 * (l + h);
+    }
+}
+
+}  // namespace
+
+// ======================= device-resident entry points ======================
+
+extern "C" int sgpu_lap_plus_diag_dev(sgpu_context *ctx, double a, double b, double c,
+                                      const sgpu_block *x, sgpu_block *y)
+{
+    if (!ctx) return SGPU_ERR_ARG;
+    SGPU_CHECK(check_pair(ctx, x, y, "sgpu_lap_plus_diag_dev"));
+    SGPU_HIP(hipSetDevice(ctx->device));
+    const bool diag = std::fabs(b) >= 1e-14;
+    if (diag && !ctx->has_veff) {
+        sgpu_set_error("sgpu_lap_plus_diag_dev: diagonal requested before sgpu_set_veff");
+        return SGPU_ERR_STATE;
+    }
+    SGPU_CHECK(sgpu_launch_stencil(ctx, x->dtype, x->ncol, a, b, c,
+                                   diag ? veff_for(ctx, x->dtype) : nullptr, x->ptr, x->ld, y->ptr,
+                                   y->ld, nullptr, 0, 1.0, 0.0, true));
+    return SGPU_OK;
+}
+
+extern "C" int sgpu_hamiltonian_dev(sgpu_context *ctx, double c, const sgpu_block *x,
+                                    sgpu_block *Hx)
+{
+    if (!ctx) return SGPU_ERR_ARG;
+    SGPU_CHECK(check_pair(ctx, x, Hx, "sgpu_hamiltonian_dev"));
+    SGPU_HIP(hipSetDevice(ctx->device));
+    return apply_h(ctx, x->dtype, x->ncol, c, x->ptr, x->ld, Hx->ptr, Hx->ld, nullptr, 0, 1.0, 0.0);
+}
+
+extern "C" int sgpu_chebyshev_filter_dev(sgpu_context *ctx, sgpu_block *X, sgpu_block *Y, int m,
+                                         double a, double b, double a0)
+{
+    if (!ctx) return SGPU_ERR_ARG;
+    SGPU_CHECK(check_pair(ctx, X, Y, "sgpu_chebyshev_filter_dev"));
+    if (m < 1 || !(b > a) || a0 == 0.5 * (a + b)) {
+        sgpu_set_error("sgpu_chebyshev_filter_dev: bad degree or bounds");
+        return SGPU_ERR_ARG;
+    }
+    SGPU_HIP(hipSetDevice(ctx->device));
+    void *res = nullptr, *prev = nullptr;
+    SGPU_CHECK(filter_raw(ctx, X->dtype, X->ncol, X->ptr, Y->ptr, X->ld, m, a, b, a0, &res, &prev));
+    Y->ptr = res;   // storage exchange instead of a copy when m is even
+    X->ptr = prev;
+    return SGPU_OK;
+}
+
+extern "C" int sgpu_project_dev(sgpu_context *ctx, const sgpu_block *Y, sgpu_block *HY,
+                                double *Hp, double *Mp)
+{
+    if (!ctx || !Hp || !Mp) return SGPU_ERR_ARG;
+    SGPU_CHECK(check_pair(ctx, Y, HY, "sgpu_project_dev"));
+    SGPU_HIP(hipSetDevice(ctx->device));
+    return project_raw(ctx, Y->dtype, Y->ncol, Y->ptr, HY->ptr, Y->ld, Hp, Mp);
+}
+
+extern "C" int sgpu_rotate_dev(sgpu_context *ctx, const sgpu_block *Y, const double *Q,
+                               sgpu_block *out)
+{
+    if (!ctx || !Q) return SGPU_ERR_ARG;
+    SGPU_CHECK(check_pair(ctx, Y, out, "sgpu_rotate_dev"));
+    if (Y->ptr == out->ptr) {
+        sgpu_set_error("sgpu_rotate_dev: output must not alias input");
+        return SGPU_ERR_ARG;
+    }
+    SGPU_HIP(hipSetDevice(ctx->device));
+    return rotate_raw(ctx, Y->dtype, Y->ncol, Y->ptr, Y->ld, Q, out->ptr, out->ld);
+}
+
+extern "C" int sgpu_chefsi_dev(sgpu_context *ctx, sgpu_block *X, sgpu_block *Y, int m, double a,
+                               double b, double a0, double *lambda)
+{
+    if (!ctx || !lambda) return SGPU_ERR_ARG;
+    SGPU_CHECK(sgpu_chebyshev_filter_dev(ctx, X, Y, m, a, b, a0));
+    const int ns = Y->ncol;
+    std::vector<double> Hp((size_t)ns * ns), Mp((size_t)ns * ns);
+    SGPU_CHECK(sgpu_project_dev(ctx, Y, X, Hp.data(), Mp.data()));
+    SGPU_CHECK(eigensolve(ctx, ns, Hp.data(), Mp.data(), lambda));
+    return sgpu_rotate_dev(ctx, Y, Hp.data(), X);
+}
+
+extern "C" int sgpu_lanczos(sgpu_context *ctx, const double *x0, double tol_min, double tol_max,
+                            int maxit, double *eigmin, double *eigmax, int *iters)
+{
+    if (!ctx || !x0 || !eigmin || !eigmax || maxit < 1) return SGPU_ERR_ARG;
+    SGPU_HIP(hipSetDevice(ctx->device));
+    const long n = ctx->nd;
+    void *buf = nullptr;
+    // separate allocation: apply_h may use the shared scratch
+    double *V = nullptr;
+    SGPU_HIP(hipMalloc((void **)&V, 3 * ctx->ld * sizeof(double)));
+    (void)buf;
+    double *Vj = V, *Vjm1 = V + ctx->ld, *Vjp1 = V + 2 * ctx->ld;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    std::vector<double> a(maxit + 2), bb(maxit + 2);
+    int status = SGPU_OK, j = 0;
+    auto run = [&]() -> int {
+        SGPU_HIP(hipMemcpyAsync(Vjm1, x0, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        double nrm2;
+        const double *pa[1] = {Vjm1}, *pb[1] = {Vjm1};
+        SGPU_CHECK(sgpu_dot_many(ctx, (int)n, 1, pa, pb, &nrm2));
+        vscale<<<nb, 256, 0, ctx->stream>>>(Vjm1, Vjm1, 1.0 / std::sqrt(nrm2), n);
+        SGPU_CHECK(apply_h(ctx, SGPU_F64, 1, 0.0, Vjm1, ctx->ld, Vj, ctx->ld, nullptr, 0, 1.0, 0.0));
+        pa[0] = Vjm1; pb[0] = Vj;
+        SGPU_CHECK(sgpu_dot_many(ctx, (int)n, 1, pa, pb, &a[0]));
+        vaxpy<<<nb, 256, 0, ctx->stream>>>(Vj, Vjm1, -a[0], n);
+        pa[0] = Vj; pb[0] = Vj;
+        SGPU_CHECK(sgpu_dot_many(ctx, (int)n, 1, pa, pb, &nrm2));
+        bb[0] = std::sqrt(nrm2);
+        vscale<<<nb, 256, 0, ctx->stream>>>(Vj, Vj, bb[0] == 0.0 ? 1.0 : 1.0 / bb[0], n);
+        double emin_pre = 0.0, emax_pre = 0.0, err_min = tol_min + 1.0, err_max = tol_max + 1.0;
+        *eigmin = *eigmax = 0.0;
+        while ((err_min > tol_min || err_max > tol_max) && j < maxit) {
+            SGPU_CHECK(apply_h(ctx, SGPU_F64, 1, 0.0, Vj, ctx->ld, Vjp1, ctx->ld, nullptr, 0, 1.0, 0.0));
+            pa[0] = Vj; pb[0] = Vjp1;
+            SGPU_CHECK(sgpu_dot_many(ctx, (int)n, 1, pa, pb, &a[j + 1]));
+            lz_orth<<<nb, 256, 0, ctx->stream>>>(Vjp1, Vj, Vjm1, a[j + 1], bb[j], n);
+            pa[0] = Vjp1; pb[0] = Vjp1;
+            SGPU_CHECK(sgpu_dot_many(ctx, (int)n, 1, pa, pb, &nrm2));
+            bb[j + 1] = std::sqrt(nrm2);
+            if (bb[j + 1] == 0.0) break;
+            vscale<<<nb, 256, 0, ctx->stream>>>(Vj, Vjp1, 1.0 / bb[j + 1], n);
+            tridiag_extremes(j + 2, a.data(), bb.data(), eigmin, eigmax);
+            err_min = std::fabs(*eigmin - emin_pre);
+            err_max = std::fabs(*eigmax - emax_pre);
+            emin_pre = *eigmin;
+            emax_pre = *eigmax;
+            j++;
+        }
+        SGPU_HIP(hipGetLastError());
+        return SGPU_OK;
+    };
+    status = run();
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(V);
+    if (iters) *iters = j;
+    return status;
+}
+
+// ======================= host-pointer drop-ins ==============================
+
+namespace {
+
+// Temporary device blocks for the host-pointer entry points.
+struct tmp_blocks {
+    std::vector<sgpu_block *> b;
+    ~tmp_blocks()
+    {
+        for (auto *p : b) sgpu_block_destroy(p);
+    }
+    int make(sgpu_context *ctx, int ncol, int dtype, sgpu_block **out)
+    {
+        SGPU_CHECK(sgpu_block_create(ctx, ncol, dtype, out));
+        b.push_back(*out);
+        return SGPU_OK;
+    }
+};
+
+int check_host_args(const sgpu_context *ctx, int ncol, const void *x, int ldi, const void *y,
+                    int ldo, const char *what)
+{
+    if (!ctx || ncol < 0 || (ncol > 0 && (!x || !y)) || ldi < ctx->nd || ldo < ctx->nd) {
+        sgpu_set_error("%s: bad arguments (ncol %d, ldi %d, ldo %d, Nd %ld)", what, ncol, ldi, ldo,
+                       ctx ? ctx->nd : 0L);
+        return SGPU_ERR_ARG;
+    }
+    return SGPU_OK;
+}
+
+}  // namespace
+
+extern "C" int sgpu_Lap_plus_diag_vec_mult(sgpu_context *ctx, int ncol, double a, double b,
+                                           double c, const double *v, const double *x, int ldi,
+                                           double *y, int ldo)
+{
+    SGPU_CHECK(check_host_args(ctx, ncol, x, ldi, y, ldo, "sgpu_Lap_plus_diag_vec_mult"));
+    if (ncol == 0) return SGPU_OK;
+    SGPU_HIP(hipSetDevice(ctx->device));
+    tmp_blocks t;
+    sgpu_block *bx, *by;
+    SGPU_CHECK(t.make(ctx, ncol, SGPU_F64, &bx));
+    SGPU_CHECK(t.make(ctx, ncol, SGPU_F64, &by));
+    SGPU_CHECK(sgpu_block_upload(bx, x, ldi, ncol));
+    const bool diag = v != nullptr && std::fabs(b) >= 1e-14;
+    double *dv = nullptr;
+    if (diag) {
+        void *s = nullptr;
+        SGPU_CHECK(sgpu_scratch(ctx, ctx->nd * sizeof(double), &s));
+        dv = (double *)s;
+        SGPU_HIP(hipMemcpyAsync(dv, v, ctx->nd * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    }
+    SGPU_CHECK(sgpu_launch_stencil(ctx, SGPU_F64, ncol, a, diag ? b : 0.0, c, dv, bx->ptr, bx->ld,
+                                   by->ptr, by->ld, nullptr, 0, 1.0, 0.0, true));
+    return sgpu_block_download(by, y, ldo, ncol);
+}
+
+extern "C" int sgpu_Lap_vec_mult(sgpu_context *ctx, int ncol, double c, const double *x, int ldi,
+                                 double *Lapx, int ldo)
+{
+    return sgpu_Lap_plus_diag_vec_mult(ctx, ncol, 1.0, 0.0, c, nullptr, x, ldi, Lapx, ldo);
+}
+
+extern "C" int sgpu_Vnl_vec_mult(sgpu_context *ctx, int ncol, const double *x, int ldi, double *Hx,
+                                 int ldo)
+{
+    SGPU_CHECK(check_host_args(ctx, ncol, x, ldi, Hx, ldo, "sgpu_Vnl_vec_mult"));
+    if (ncol == 0) return SGPU_OK;
+    SGPU_HIP(hipSetDevice(ctx->device));
+    tmp_blocks t;
+    sgpu_block *bx, *by;
+    SGPU_CHECK(t.make(ctx, ncol, SGPU_F64, &bx));
+    SGPU_CHECK(t.make(ctx, ncol, SGPU_F64, &by));
+    SGPU_CHECK(sgpu_block_upload(bx, x, ldi, ncol));
+    SGPU_CHECK(sgpu_block_upload(by, Hx, ldo, ncol));
+    SGPU_CHECK(sgpu_launch_vnl(ctx, SGPU_F64, ncol, bx->ptr, bx->ld, by->ptr, by->ld, 1.0));
+    return sgpu_block_download(by, Hx, ldo, ncol);
+}
+
+extern "C" int sgpu_Hamiltonian_vectors_mult(sgpu_context *ctx, int ncol, double c, const double *x,
+                                             int ldi, double *Hx, int ldo)
+{
+    SGPU_CHECK(check_host_args(ctx, ncol, x, ldi, Hx, ldo, "sgpu_Hamiltonian_vectors_mult"));
+    if (ncol == 0) return SGPU_OK;
+    SGPU_HIP(hipSetDevice(ctx->device));
+    tmp_blocks t;
+    sgpu_block *bx, *by;
+    SGPU_CHECK(t.make(ctx, ncol, SGPU_F64, &bx));
+    SGPU_CHECK(t.make(ctx, ncol, SGPU_F64, &by));
+    SGPU_CHECK(sgpu_block_upload(bx, x, ldi, ncol));
+    SGPU_CHECK(sgpu_hamiltonian_dev(ctx, c, bx, by));
+    return sgpu_block_download(by, Hx, ldo, ncol);
+}
+
+extern "C" int sgpu_ChebyshevFiltering(sgpu_context *ctx, double *X, int ldi, double *Y, int ldo,
+                                       int ncol, int m, double a, double b, double a0, int dtype)
+{
+    SGPU_CHECK(check_host_args(ctx, ncol, X, ldi, Y, ldo, "sgpu_ChebyshevFiltering"));
+    if (dtype != SGPU_F64 && dtype != SGPU_F32) return SGPU_ERR_ARG;
+    if (ncol == 0) return SGPU_OK;
+    SGPU_HIP(hipSetDevice(ctx->device));
+    tmp_blocks t;
+    sgpu_block *bx, *by;
+    SGPU_CHECK(t.make(ctx, ncol, dtype, &bx));
+    SGPU_CHECK(t.make(ctx, ncol, dtype, &by));
+    SGPU_CHECK(sgpu_block_upload(bx, X, ldi, ncol));
+    SGPU_CHECK(sgpu_chebyshev_filter_dev(ctx, bx, by, m, a, b, a0));
+    SGPU_CHECK(sgpu_block_download(by, Y, ldo, ncol));
+    return sgpu_block_download(bx, X, ldi, ncol);
+}
+
+extern "C" int sgpu_DP_Project_Hamiltonian(sgpu_context *ctx, const double *Y, int ldi, double *HY,
+                                           int ldo, int ncol, double *Hp, double *Mp, int dtype)
+{
+    SGPU_CHECK(check_host_args(ctx, ncol, Y, ldi, HY, ldo, "sgpu_DP_Project_Hamiltonian"));
+    if ((dtype != SGPU_F64 && dtype != SGPU_F32) || !Hp || !Mp) return SGPU_ERR_ARG;
+    if (ncol == 0) return SGPU_OK;
+    SGPU_HIP(hipSetDevice(ctx->device));
+    tmp_blocks t;
+    sgpu_block *by, *bh;
+    SGPU_CHECK(t.make(ctx, ncol, dtype, &by));
+    SGPU_CHECK(t.make(ctx, ncol, dtype, &bh));
+    SGPU_CHECK(sgpu_block_upload(by, Y, ldi, ncol));
+    SGPU_CHECK(sgpu_project_dev(ctx, by, bh, Hp, Mp));
+    return sgpu_block_download(bh, HY, ldo, ncol);
+}
+
+extern "C" int sgpu_DP_Solve_Generalized_EigenProblem(sgpu_context *ctx, int n, double *Hp,
+                                                      double *Mp, double *lambda)
+{
+    if (!ctx || n <= 0 || !Hp || !Mp || !lambda) {
+        sgpu_set_error("sgpu_DP_Solve_Generalized_EigenProblem: bad arguments");
+        return SGPU_ERR_ARG;
+    }
+    SGPU_HIP(hipSetDevice(ctx->device));
+    return eigensolve(ctx, n, Hp, Mp, lambda);
+}
+
+extern "C" int sgpu_DP_Subspace_Rotation(sgpu_context *ctx, const double *Y, int ldy,
+                                         const double *Q, int ns, double *Psi_rot, int ldo,
+                                         int dtype)
+{
+    SGPU_CHECK(check_host_args(ctx, ns, Y, ldy, Psi_rot, ldo, "sgpu_DP_Subspace_Rotation"));
+    if ((dtype != SGPU_F64 && dtype != SGPU_F32) || !Q) return SGPU_ERR_ARG;
+    if (ns == 0) return SGPU_OK;
+    SGPU_HIP(hipSetDevice(ctx->device));
+    tmp_blocks t;
+    sgpu_block *by, *bo;
+    SGPU_CHECK(t.make(ctx, ns, dtype, &by));
+    SGPU_CHECK(t.make(ctx, ns, dtype, &bo));
+    SGPU_CHECK(sgpu_block_upload(by, Y, ldy, ns));
+    SGPU_CHECK(sgpu_rotate_dev(ctx, by, Q, bo));
+    return sgpu_block_download(bo, Psi_rot, ldo, ns);
+}

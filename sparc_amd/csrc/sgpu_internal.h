@@ -1,0 +1,149 @@
+// Internal definitions shared by the libsparcgpu.so translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "sparc_gpu.h"
+
+#define SGPU_MAX_FDN 8
+
+// Thread-local last error message (sgpu_last_error).
+void sgpu_set_error(const char *fmt, ...);
+
+#define SGPU_HIP(call)                                                              \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            sgpu_set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,               \
+                           hipGetErrorString(e_));                                  \
+            return SGPU_ERR_HIP;                                                    \
+        }                                                                           \
+    } while (0)
+
+#define SGPU_BLAS(call)                                                             \
+    do {                                                                            \
+        rocblas_status s_ = (call);                                                 \
+        if (s_ != rocblas_status_success) {                                         \
+            sgpu_set_error("%s:%d %s: rocblas status %d", __FILE__, __LINE__, #call, \
+                           (int)s_);                                                \
+            return SGPU_ERR_HIP;                                                    \
+        }                                                                           \
+    } while (0)
+
+#define SGPU_CHECK(expr)                 \
+    do {                                 \
+        int st_ = (expr);                \
+        if (st_ < 0) return st_;         \
+    } while (0)
+
+struct sgpu_block {
+    sgpu_context *ctx;
+    int ncol;
+    int dtype;   // SGPU_F64 / SGPU_F32
+    long ld;     // elements between columns (>= Nd, padded to 64)
+    void *ptr;   // device storage, ncol * ld elements
+};
+
+// Device-side nonlocal projector set (see sgpu_set_nonlocal).
+struct sgpu_nonlocal {
+    int n_infl = 0;
+    int n_alpha = 0;
+    int max_np = 0;
+    long total_pts = 0;
+    int *ndc = nullptr, *nproj = nullptr, *alpha_off = nullptr;
+    long *pos_off = nullptr, *chi_off = nullptr;
+    int *grid_pos = nullptr;
+    double *chi64 = nullptr;
+    float *chi32 = nullptr;
+    double *gamma = nullptr;
+    // point-centric inverse map: for grid point q, entries inv_ptr[q]..inv_ptr[q+1]-1
+    // give (influence entry, local index within that entry's ndc points).
+    int *inv_ptr = nullptr;
+    int *inv_ent = nullptr;
+    int *inv_loc = nullptr;
+    int *cov_pts = nullptr;   // grid points with at least one projector entry
+    int n_cov = 0;
+    // per-entry partial rows (entry a owns rows part_row[a] .. + nproj[a]) and,
+    // for each alpha row r, the partial rows arow_src[arow_ptr[r] ..] summed
+    // into it (periodic images of one atom share alpha rows)
+    int *part_row = nullptr;
+    int n_part = 0;
+    int *arow_ptr = nullptr;
+    int *arow_src = nullptr;
+    double dV = 0.0;
+};
+
+struct sgpu_timer_acc {
+    double total_ms = 0.0;
+    int count = 0;
+};
+
+struct sgpu_context {
+    int device = 0;
+    int nx = 0, ny = 0, nz = 0;
+    long nd = 0;
+    long ld = 0;  // padded column stride used by device blocks
+    double hx = 0, hy = 0, hz = 0;
+    int bcx = 0, bcy = 0, bcz = 0;
+    int fdn = 0;
+    double d2x[SGPU_MAX_FDN + 1] = {0}, d2y[SGPU_MAX_FDN + 1] = {0}, d2z[SGPU_MAX_FDN + 1] = {0};
+    hipStream_t stream = nullptr;
+    rocblas_handle blas = nullptr;
+
+    double *veff64 = nullptr;
+    float *veff32 = nullptr;
+    bool has_veff = false;
+
+    sgpu_nonlocal nl;
+
+    // scratch (grown on demand, owned by the context)
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    double *red_buf = nullptr;      // reduction partials
+    size_t red_elems = 0;
+    double *host_red = nullptr;     // pinned host mirror of small results
+
+    bool timing = false;
+    std::map<std::string, sgpu_timer_acc> timers;
+    struct pending_pair {
+        hipEvent_t a, b;
+        std::string name;
+    };
+    std::vector<pending_pair> pending;
+    std::vector<hipEvent_t> ev_pool;
+    // live blocks; sgpu_destroy releases their storage and orphans them
+    std::vector<sgpu_block *> blocks;
+};
+
+// ---- helpers implemented in sgpu_core.hip ----
+int sgpu_scratch(sgpu_context *ctx, size_t bytes, void **p);
+int sgpu_reduce_buf(sgpu_context *ctx, size_t elems, double **p);
+size_t sgpu_dtype_size(int dtype);
+
+// RAII-less event timer: begin returns a token, end accumulates.
+struct sgpu_tmark {
+    hipEvent_t a = nullptr, b = nullptr;
+};
+int sgpu_tbegin(sgpu_context *ctx, sgpu_tmark *m);
+int sgpu_tend(sgpu_context *ctx, sgpu_tmark *m, const char *name);
+int sgpu_timer_resolve(sgpu_context *ctx);
+
+// ---- kernel launchers (stencil.hip) ----
+// out = s1 * ((a Lap + bdiag diag(v) + c) x) + s2 * z, per column (z may be null).
+int sgpu_launch_stencil(sgpu_context *ctx, int dtype, int ncol, double a, double bdiag, double c,
+                        const void *v, const void *x, long ldx, void *out, long ldo,
+                        const void *z, long ldz, double s1, double s2, bool allow_tiled);
+// ---- nonlocal.hip ----
+// out += scale * Vnl x  (alpha computed from x); dtype of x/out.
+int sgpu_launch_vnl(sgpu_context *ctx, int dtype, int ncol, const void *x, long ldx, void *out,
+                    long ldo, double scale);
+// ---- blas.hip ----
+int sgpu_dot_many(sgpu_context *ctx, int n, int npairs, const double *const *a,
+                  const double *const *b, double *host_out);
+int sgpu_convert(sgpu_context *ctx, const void *src, int st, void *dst, int dt, long n);

@@ -1,0 +1,270 @@
+// Order-2R finite-difference stencil kernels with fused diagonal, shift and
+// three-term (Chebyshev) update:
+//
+//   out = s1 * ((a Lap + bdiag diag(v) + c) x) + s2 * z          (per column)
+//
+// This is the arithmetic of Lap_plus_diag_vec_mult_orth +
+// stencil_3axis_thread_radius6 (SPARC src/lapVecRoutines.c:185-227,306-611)
+// with the scaling/recurrence of ChebyshevFiltering (src/eigenSolver.c:763-794)
+// folded into the epilogue, so one pass reads x, v, z once and writes out once.
+// Periodic halos wrap indices (the reference copies the opposite face,
+// :536-577); Dirichlet halos read zero.
+//
+// Two kernels:
+//  * stencil_tiled<T, TX, TY, RY>: radius 6.  A TX x TY tile of the x-y plane
+//    (plus a 6-deep halo) is staged in LDS per z-plane while the block marches
+//    along z; each thread keeps a 13-deep register queue of its own points'
+//    z-neighbours, so every x value is read from HBM about once.
+//  * stencil_direct<T, R>: one thread per point, all 6R+1 loads from global;
+//    used for other FD orders and as the independent cross-check.
+#include "sgpu_internal.h"
+
+namespace {
+
+template <typename T>
+struct StencilParams {
+    const T *x;
+    T *out;
+    const T *z;
+    const T *v;
+    long ldx, ldo, ldz;
+    int nx, ny, nz;
+    int bcx, bcy, bcz;
+    T wx[SGPU_MAX_FDN + 1], wy[SGPU_MAX_FDN + 1], wz[SGPU_MAX_FDN + 1];
+    T w0;     // a (D2x0 + D2y0 + D2z0) + c
+    T bdiag;  // coefficient of v (0 -> no diagonal term)
+    T s1, s2;
+};
+
+// Index along one axis with the halo convention; returns -1 for a Dirichlet
+// point outside the domain.
+__device__ __forceinline__ int axis_index(int g, int n, int bc)
+{
+    if (g >= 0 && g < n) return g;
+    if (bc) return -1;
+    int w = g % n;
+    return w < 0 ? w + n : w;
+}
+
+template <typename T, int R>
+__global__ __launch_bounds__(256) void stencil_direct(StencilParams<T> p)
+{
+    const long nxny = (long)p.nx * p.ny;
+    const long nd = nxny * p.nz;
+    const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= nd) return;
+    const int col = blockIdx.y;
+    const int k = (int)(idx / nxny);
+    const long rem = idx - (long)k * nxny;
+    const int j = (int)(rem / p.nx);
+    const int i = (int)(rem - (long)j * p.nx);
+    const T *x = p.x + col * p.ldx;
+    const T xc = x[idx];
+    T res = p.w0 * xc;
+#pragma unroll
+    for (int r = 1; r <= R; r++) {
+        int a, b;
+        a = axis_index(i - r, p.nx, p.bcx);
+        b = axis_index(i + r, p.nx, p.bcx);
+        T xm = a >= 0 ? x[k * nxny + (long)j * p.nx + a] : T(0);
+        T xp = b >= 0 ? x[k * nxny + (long)j * p.nx + b] : T(0);
+        T rx = (xm + xp) * p.wx[r];
+        a = axis_index(j - r, p.ny, p.bcy);
+        b = axis_index(j + r, p.ny, p.bcy);
+        xm = a >= 0 ? x[k * nxny + (long)a * p.nx + i] : T(0);
+        xp = b >= 0 ? x[k * nxny + (long)b * p.nx + i] : T(0);
+        T ry = (xm + xp) * p.wy[r];
+        a = axis_index(k - r, p.nz, p.bcz);
+        b = axis_index(k + r, p.nz, p.bcz);
+        xm = a >= 0 ? x[a * nxny + (long)j * p.nx + i] : T(0);
+        xp = b >= 0 ? x[b * nxny + (long)j * p.nx + i] : T(0);
+        T rz = (xm + xp) * p.wz[r];
+        res += rx + ry + rz;
+    }
+    if (p.bdiag != T(0)) res += p.bdiag * (p.v[idx] * xc);
+    T o = p.s1 * res;
+    if (p.z) o += p.s2 * p.z[col * p.ldz + idx];
+    p.out[col * p.ldo + idx] = o;
+}
+
+constexpr int kR = 6;
+
+template <typename T, int TX, int TY, int RY>
+__global__ __launch_bounds__(TX *(TY / RY)) void stencil_tiled(StencilParams<T> p, int zchunk)
+{
+    constexpr int NTY = TY / RY;
+    constexpr int NT = TX * NTY;
+    constexpr int LX = TX + 2 * kR;
+    constexpr int LY = TY + 2 * kR;
+    constexpr int NHALO = 2 * kR * TY + 2 * kR * TX;
+    constexpr int HPT = (NHALO + NT - 1) / NT;
+    __shared__ T tile[LY][LX];
+
+    const int tilesx = (p.nx + TX - 1) / TX;
+    const int bx = blockIdx.x % tilesx;
+    const int by = blockIdx.x / tilesx;
+    const int col = blockIdx.y;
+    const int zs = blockIdx.z * zchunk;
+    const int ze = min(p.nz, zs + zchunk);
+    const int tx = threadIdx.x % TX;
+    const int tyb = threadIdx.x / TX;
+    const int i0 = bx * TX, j0 = by * TY;
+    const long nxny = (long)p.nx * p.ny;
+    const T *x = p.x + col * p.ldx;
+
+    // in-plane offsets of my interior points (-1: Dirichlet zero)
+    long coff[RY];
+    long ooff[RY];  // output offset, -1 if outside the grid
+#pragma unroll
+    for (int r = 0; r < RY; r++) {
+        const int ty = tyb + r * NTY;
+        const int ig = i0 + tx, jg = j0 + ty;
+        const int ia = axis_index(ig, p.nx, p.bcx), ja = axis_index(jg, p.ny, p.bcy);
+        coff[r] = (ia >= 0 && ja >= 0) ? (long)ja * p.nx + ia : -1;
+        ooff[r] = (ig < p.nx && jg < p.ny) ? (long)jg * p.nx + ig : -1;
+    }
+    // my share of the halo: LDS slot and in-plane offset
+    int hslot[HPT];
+    long hoff[HPT];
+#pragma unroll
+    for (int h = 0; h < HPT; h++) {
+        const int e = threadIdx.x + h * NT;
+        hslot[h] = -1;
+        hoff[h] = -1;
+        if (e < NHALO) {
+            int ly, lx;
+            if (e < 2 * kR * TY) {  // x halo: left/right strips of the interior rows
+                const int row = e / (2 * kR), c = e % (2 * kR);
+                ly = kR + row;
+                lx = c < kR ? c : TX + c;
+            } else {  // y halo: top/bottom strips of the interior columns
+                const int e2 = e - 2 * kR * TY;
+                const int rr = e2 / TX, c = e2 % TX;
+                ly = rr < kR ? rr : TY + rr;
+                lx = kR + c;
+            }
+            hslot[h] = ly * LX + lx;
+            const int ia = axis_index(i0 - kR + lx, p.nx, p.bcx);
+            const int ja = axis_index(j0 - kR + ly, p.ny, p.bcy);
+            hoff[h] = (ia >= 0 && ja >= 0) ? (long)ja * p.nx + ia : -1;
+        }
+    }
+
+    T q[RY][2 * kR + 1];
+    // warm-up: planes zs-R .. zs+R-1 into q[.][1..2R]
+#pragma unroll
+    for (int s = 0; s < 2 * kR; s++) {
+        const int za = axis_index(zs - kR + s, p.nz, p.bcz);
+#pragma unroll
+        for (int r = 0; r < RY; r++)
+            q[r][s + 1] = (za >= 0 && coff[r] >= 0) ? x[za * nxny + coff[r]] : T(0);
+    }
+
+    for (int z = zs; z < ze; z++) {
+        const int zn = axis_index(z + kR, p.nz, p.bcz);
+#pragma unroll
+        for (int r = 0; r < RY; r++) {
+#pragma unroll
+            for (int s = 0; s < 2 * kR; s++) q[r][s] = q[r][s + 1];
+            q[r][2 * kR] = (zn >= 0 && coff[r] >= 0) ? x[zn * nxny + coff[r]] : T(0);
+        }
+        const long zb = (long)z * nxny;
+        T hv[HPT];
+#pragma unroll
+        for (int h = 0; h < HPT; h++) hv[h] = (hoff[h] >= 0) ? x[zb + hoff[h]] : T(0);
+        __syncthreads();  // previous plane fully consumed
+#pragma unroll
+        for (int r = 0; r < RY; r++) tile[kR + tyb + r * NTY][kR + tx] = q[r][kR];
+#pragma unroll
+        for (int h = 0; h < HPT; h++)
+            if (hslot[h] >= 0) (&tile[0][0])[hslot[h]] = hv[h];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RY; r++) {
+            if (ooff[r] < 0) continue;
+            const int ly = kR + tyb + r * NTY, lx = kR + tx;
+            const T xc = q[r][kR];
+            T res = p.w0 * xc;
+#pragma unroll
+            for (int s = 1; s <= kR; s++) {
+                T rx = (tile[ly][lx - s] + tile[ly][lx + s]) * p.wx[s];
+                T ry = (tile[ly - s][lx] + tile[ly + s][lx]) * p.wy[s];
+                T rz = (q[r][kR - s] + q[r][kR + s]) * p.wz[s];
+                res += rx + ry + rz;
+            }
+            const long o = zb + ooff[r];
+            if (p.bdiag != T(0)) res += p.bdiag * (p.v[o] * xc);
+            T val = p.s1 * res;
+            if (p.z) val += p.s2 * p.z[col * p.ldz + o];
+            p.out[col * p.ldo + o] = val;
+        }
+    }
+}
+
+template <typename T>
+int launch_typed(sgpu_context *ctx, int ncol, double a, double bdiag, double c, const T *v,
+                 const T *x, long ldx, T *out, long ldo, const T *z, long ldz, double s1,
+                 double s2, bool allow_tiled)
+{
+    StencilParams<T> p;
+    p.x = x; p.out = out; p.z = z; p.v = v;
+    p.ldx = ldx; p.ldo = ldo; p.ldz = ldz;
+    p.nx = ctx->nx; p.ny = ctx->ny; p.nz = ctx->nz;
+    p.bcx = ctx->bcx; p.bcy = ctx->bcy; p.bcz = ctx->bcz;
+    for (int r = 0; r <= SGPU_MAX_FDN; r++) {
+        p.wx[r] = (T)(r <= ctx->fdn ? ctx->d2x[r] * a : 0.0);
+        p.wy[r] = (T)(r <= ctx->fdn ? ctx->d2y[r] * a : 0.0);
+        p.wz[r] = (T)(r <= ctx->fdn ? ctx->d2z[r] * a : 0.0);
+    }
+    p.w0 = (T)(a * (ctx->d2x[0] + ctx->d2y[0] + ctx->d2z[0]) + c);
+    p.bdiag = (v && std::fabs(bdiag) >= 1e-14) ? (T)bdiag : T(0);
+    p.s1 = (T)s1;
+    p.s2 = (T)s2;
+    if (p.bdiag != T(0) && !v) {
+        sgpu_set_error("stencil: diagonal requested without Veff");
+        return SGPU_ERR_STATE;
+    }
+    if (ncol <= 0) return SGPU_OK;
+    if (ncol > 65535) {
+        sgpu_set_error("stencil: ncol %d exceeds the launch limit", ncol);
+        return SGPU_ERR_ARG;
+    }
+    if (allow_tiled && ctx->fdn == kR) {
+        constexpr int TX = 32, TY = 16, RY = 2;
+        const int tiles = ((ctx->nx + TX - 1) / TX) * ((ctx->ny + TY - 1) / TY);
+        long want = (2048 + (long)tiles * ncol - 1) / ((long)tiles * ncol);
+        int nzc = (int)std::max(1L, std::min<long>(want, std::max(1, ctx->nz / 16)));
+        int zchunk = (ctx->nz + nzc - 1) / nzc;
+        nzc = (ctx->nz + zchunk - 1) / zchunk;
+        dim3 grid(tiles, ncol, nzc);
+        stencil_tiled<T, TX, TY, RY><<<grid, TX * (TY / RY), 0, ctx->stream>>>(p, zchunk);
+    } else {
+        dim3 grid((unsigned)((ctx->nd + 255) / 256), ncol);
+        switch (ctx->fdn) {
+            case 1: stencil_direct<T, 1><<<grid, 256, 0, ctx->stream>>>(p); break;
+            case 2: stencil_direct<T, 2><<<grid, 256, 0, ctx->stream>>>(p); break;
+            case 3: stencil_direct<T, 3><<<grid, 256, 0, ctx->stream>>>(p); break;
+            case 4: stencil_direct<T, 4><<<grid, 256, 0, ctx->stream>>>(p); break;
+            case 5: stencil_direct<T, 5><<<grid, 256, 0, ctx->stream>>>(p); break;
+            case 6: stencil_direct<T, 6><<<grid, 256, 0, ctx->stream>>>(p); break;
+            case 7: stencil_direct<T, 7><<<grid, 256, 0, ctx->stream>>>(p); break;
+            default: stencil_direct<T, 8><<<grid, 256, 0, ctx->stream>>>(p); break;
+        }
+    }
+    SGPU_HIP(hipGetLastError());
+    return SGPU_OK;
+}
+
+}  // namespace
+
+int sgpu_launch_stencil(sgpu_context *ctx, int dtype, int ncol, double a, double bdiag, double c,
+                        const void *v, const void *x, long ldx, void *out, long ldo, const void *z,
+                        long ldz, double s1, double s2, bool allow_tiled)
+{
+    if (dtype == SGPU_F64)
+        return launch_typed<double>(ctx, ncol, a, bdiag, c, (const double *)v, (const double *)x,
+                                    ldx, (double *)out, ldo, (const double *)z, ldz, s1, s2,
+                                    allow_tiled);
+    return launch_typed<float>(ctx, ncol, a, bdiag, c, (const float *)v, (const float *)x, ldx,
+                               (float *)out, ldo, (const float *)z, ldz, s1, s2, allow_tiled);
+}

@@ -1,0 +1,146 @@
+"""GPU parity of the Chebyshev filter, Rayleigh-Ritz pieces, Lanczos and a
+full CheFSI iteration against the FP64 oracle (through the C ABI)."""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from tests.helpers import rel_err, small_case
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from sparc_amd import gpu
+    gpu.load()
+    return gpu
+
+
+def _bounds(o, g, seed=11):
+    from sparc_amd.workloads import uniform_block
+    x0 = uniform_block(g.nd, 1, seed, 0.0, 1.0)[:, 0]
+    emin, emax, _ = o.lanczos(x0, 1e-2, 1e-2, 1000)
+    emax *= 1.01
+    emin -= 0.1
+    return emin, emax, 0.5 * (emin + emax)
+
+
+@pytest.mark.parametrize("m", [1, 2, 7, 20])
+def test_filter_fp64(gpu, oracle_mod, m):
+    g, veff, nl, rng = small_case(n=(16, 15, 14), n_sites=3, seed=21)
+    o = oracle_mod.Oracle(g, veff, nl)
+    a0, b, a = _bounds(o, g)
+    X = rng.uniform(-0.5, 0.5, (g.nd, 10))
+    Yo, Xo = o.chebyshev_filter(X, m, a, b, a0)
+    with gpu.Context(g) as ctx:
+        ctx.set_veff(veff)
+        ctx.set_nonlocal(nl)
+        Y, Xc = ctx.ChebyshevFiltering(X, m, a, b, a0, gpu.F64)
+    assert rel_err(Y, Yo) < 1e-11
+    assert rel_err(Xc, Xo) < 1e-11
+
+
+def test_filter_fp32(gpu, oracle_mod):
+    g, veff, nl, rng = small_case(n=(20, 19, 18), n_sites=4, seed=22)
+    o = oracle_mod.Oracle(g, veff, nl)
+    a0, b, a = _bounds(o, g)
+    X = rng.uniform(-0.5, 0.5, (g.nd, 16))
+    Yo, _ = o.chebyshev_filter(X, 20, a, b, a0)
+    with gpu.Context(g) as ctx:
+        ctx.set_veff(veff)
+        ctx.set_nonlocal(nl)
+        Y, _ = ctx.ChebyshevFiltering(X, 20, a, b, a0, gpu.F32)
+    # FP32 recurrence of degree 20: tolerance stated against the FP64 oracle
+    assert rel_err(Y, Yo) < 1e-4
+
+
+def test_filter_dev_storage_exchange(gpu, oracle_mod):
+    """Even degree: the device filter swaps storage instead of copying."""
+    g, veff, nl, rng = small_case(n=(12, 12, 12), n_sites=2, seed=23)
+    o = oracle_mod.Oracle(g, veff, nl)
+    a0, b, a = _bounds(o, g)
+    X = rng.uniform(-0.5, 0.5, (g.nd, 5))
+    Yo, Xo = o.chebyshev_filter(X, 6, a, b, a0)
+    with gpu.Context(g) as ctx:
+        ctx.set_veff(veff)
+        ctx.set_nonlocal(nl)
+        bx = ctx.block(5).upload(X)
+        by = ctx.block(5)
+        ctx.chebyshev_filter_dev(bx, by, 6, a, b, a0)
+        assert rel_err(by.download(), Yo) < 1e-11
+        assert rel_err(bx.download(), Xo) < 1e-11
+
+
+@pytest.mark.parametrize("dtype,tol", [(0, 1e-12), (1, 2e-5)])
+def test_project_rotate(gpu, oracle_mod, dtype, tol):
+    g, veff, nl, rng = small_case(n=(16, 16, 15), n_sites=3, seed=24)
+    o = oracle_mod.Oracle(g, veff, nl)
+    Y = rng.uniform(-0.5, 0.5, (g.nd, 12))
+    Hpo, Mpo, HYo = o.project(Y)
+    Q = np.linalg.qr(rng.standard_normal((12, 12)))[0]
+    with gpu.Context(g) as ctx:
+        ctx.set_veff(veff)
+        ctx.set_nonlocal(nl)
+        Yin = Y.astype(np.float32).astype(np.float64) if dtype == 1 else Y
+        Hp, Mp, HY = ctx.DP_Project_Hamiltonian(Yin, dtype)
+        R = ctx.DP_Subspace_Rotation(Yin, Q, dtype)
+    assert rel_err(Mp, Mpo) < tol
+    assert rel_err(Hp, Hpo) < tol
+    assert rel_err(HY, HYo) < max(tol, 1e-12)
+    assert rel_err(R, o.rotate(Y, Q)) < max(tol, 1e-12)
+
+
+def test_generalized_eigensolve(gpu, oracle_mod):
+    rng = np.random.default_rng(25)
+    n = 40
+    A = rng.standard_normal((n, n)); Hp = A + A.T
+    B = rng.standard_normal((n, n)); Mp = B @ B.T + n * np.eye(n)
+    from sparc_amd.workloads import Grid
+    with gpu.Context(Grid(8, 8, 8, 0.5, 0.5, 0.5)) as ctx:
+        lam, Q = ctx.DP_Solve_Generalized_EigenProblem(Hp, Mp)
+    lam_ref = sla.eigh(Hp, Mp, eigvals_only=True, driver="gvd")
+    assert np.max(np.abs(lam - lam_ref)) < 1e-10
+    assert np.allclose(Q.T @ Mp @ Q, np.eye(n), atol=1e-10)
+    assert np.allclose(Hp @ Q, Mp @ Q * lam, atol=1e-9)
+    lo, _ = oracle_mod.sygv(Hp, Mp)
+    assert np.max(np.abs(lam - lo)) < 1e-10
+
+
+def test_lanczos_matches_oracle(gpu, oracle_mod):
+    from sparc_amd.workloads import uniform_block
+    g, veff, nl, rng = small_case(n=(14, 14, 14), n_sites=3, seed=26)
+    o = oracle_mod.Oracle(g, veff, nl)
+    x0 = uniform_block(g.nd, 1, 5, 0.0, 1.0)[:, 0]
+    emin_o, emax_o, it_o = o.lanczos(x0, 1e-2, 1e-2, 1000)
+    with gpu.Context(g) as ctx:
+        ctx.set_veff(veff)
+        ctx.set_nonlocal(nl)
+        emin, emax, it = ctx.Lanczos(x0, 1e-2, 1e-2, 1000)
+        e10 = ctx.Lanczos(x0, 0.0, 0.0, 10)
+    assert it == it_o
+    assert abs(emin - emin_o) < 1e-9 and abs(emax - emax_o) < 1e-9
+    assert e10[2] == 10
+    e10o = o.lanczos(x0, 0.0, 0.0, 10)
+    assert abs(e10[0] - e10o[0]) < 1e-9 and abs(e10[1] - e10o[1]) < 1e-9
+
+
+@pytest.mark.parametrize("dtype,tol", [(0, 1e-8), (1, 5e-5)])
+def test_chefsi_converges_to_dense_spectrum(gpu, oracle_mod, dtype, tol):
+    g, veff, nl, rng = small_case(n=(12, 12, 12), h=(0.5, 0.5, 0.5), n_sites=3, seed=27)
+    o = oracle_mod.Oracle(g, veff, nl)
+    H = np.diag(veff) + (-0.5) * o.lap_vec_mult(np.eye(g.nd)) + nl.dense(g.nd)
+    ev = np.linalg.eigvalsh(0.5 * (H + H.T))
+    ns = 16
+    X = rng.uniform(-0.5, 0.5, (g.nd, ns))
+    a0, b, a = _bounds(o, g)
+    with gpu.Context(g) as ctx:
+        ctx.set_veff(veff)
+        ctx.set_nonlocal(nl)
+        bx = ctx.block(ns, dtype).upload(X)
+        by = ctx.block(ns, dtype)
+        lam = None
+        for it in range(25):
+            lam = ctx.chefsi_dev(bx, by, 20, a, b, a0)
+            a0, a = lam[0], lam[-1] + 0.1
+    nconv = ns - 4
+    assert np.max(np.abs(lam[:nconv] - ev[:nconv])) < tol

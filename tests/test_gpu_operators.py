@@ -1,0 +1,110 @@
+"""GPU parity of the operator kernels against the FP64 oracle (through the C ABI)."""
+import numpy as np
+import pytest
+
+from tests.helpers import rel_err, small_case, split_images
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from sparc_amd import gpu
+    gpu.load()
+    return gpu
+
+
+CASES = [
+    ((14, 13, 12), (0, 0, 0)),   # periodic, non-cubic
+    ((14, 13, 12), (1, 1, 1)),   # Dirichlet
+    ((13, 17, 9), (0, 1, 0)),    # mixed
+    ((40, 19, 8), (0, 0, 0)),    # several x tiles, partial y tile, nz < 2*radius+...
+    ((6, 6, 6), (0, 0, 0)),      # smallest periodic grid allowed (n = radius)
+]
+
+
+@pytest.mark.parametrize("n,bc", CASES)
+def test_lap_plus_diag_fp64(gpu, oracle_mod, n, bc):
+    g, veff, nl, rng = small_case(n=n, bc=bc, seed=1)
+    o = oracle_mod.Oracle(g, veff)
+    x = rng.standard_normal((g.nd, 3))
+    with gpu.Context(g) as ctx:
+        for a, b, c in ((1.0, 0.0, 0.0), (-0.5, 1.0, -0.7), (2.0, 0.5, 3.0)):
+            y = ctx.Lap_plus_diag_vec_mult(x, a, b, c, veff)
+            yo = o.lap_plus_diag(x, a, b, c, veff)
+            assert rel_err(y, yo) < 1e-13, (a, b, c)
+
+
+@pytest.mark.parametrize("order", [2, 4, 8, 16])
+def test_lap_other_orders(gpu, oracle_mod, order):
+    g, veff, nl, rng = small_case(n=(12, 11, 10), order=order, seed=2)
+    o = oracle_mod.Oracle(g, veff)
+    x = rng.standard_normal((g.nd, 2))
+    with gpu.Context(g) as ctx:
+        y = ctx.Lap_vec_mult(x, 0.25)
+        assert rel_err(y, o.lap_vec_mult(x, 0.25)) < 1e-13
+
+
+@pytest.mark.parametrize("images", [False, True])
+def test_vnl_fp64(gpu, oracle_mod, images):
+    g, veff, nl, rng = small_case(n=(16, 15, 14), n_sites=5, rc=2.6, seed=3, lmax=2)
+    if images:
+        nl = split_images(nl, 1)
+    o = oracle_mod.Oracle(g, veff, nl)
+    x = rng.standard_normal((g.nd, 11))
+    h0 = rng.standard_normal((g.nd, 11))
+    with gpu.Context(g) as ctx:
+        ctx.set_nonlocal(nl)
+        y = ctx.Vnl_vec_mult(x, h0)
+    yo = o.vnl_vec_mult(x, h0)
+    assert rel_err(y - h0, yo - h0) < 1e-12
+
+
+def test_vnl_matches_dense_operator(gpu):
+    g, veff, nl, rng = small_case(n=(9, 8, 7), n_sites=2, rc=1.6, seed=4)
+    V = nl.dense(g.nd)
+    assert np.allclose(V, V.T)
+    x = rng.standard_normal((g.nd, 4))
+    with gpu.Context(g) as ctx:
+        ctx.set_nonlocal(nl)
+        y = ctx.Vnl_vec_mult(x)
+    assert rel_err(y, V @ x) < 1e-12
+
+
+@pytest.mark.parametrize("bc", [(0, 0, 0), (1, 1, 1)])
+def test_hamiltonian_fp64(gpu, oracle_mod, bc):
+    g, veff, nl, rng = small_case(n=(18, 17, 16), bc=bc, n_sites=4, seed=5)
+    o = oracle_mod.Oracle(g, veff, nl)
+    x = rng.standard_normal((g.nd, 7))
+    with gpu.Context(g) as ctx:
+        ctx.set_veff(veff)
+        ctx.set_nonlocal(nl)
+        y = ctx.Hamiltonian_vectors_mult(x, -0.3)
+    assert rel_err(y, o.hamiltonian(x, -0.3)) < 1e-12
+
+
+def test_hamiltonian_fp32_block(gpu, oracle_mod):
+    g, veff, nl, rng = small_case(n=(20, 18, 16), n_sites=4, seed=6)
+    o = oracle_mod.Oracle(g, veff, nl)
+    x = rng.uniform(-0.5, 0.5, (g.nd, 9))
+    with gpu.Context(g) as ctx:
+        ctx.set_veff(veff)
+        ctx.set_nonlocal(nl)
+        bx = ctx.block(9, gpu.F32).upload(x)
+        by = ctx.block(9, gpu.F32)
+        ctx.hamiltonian_dev(0.1, bx, by)
+        y = by.download()
+    assert rel_err(y, o.hamiltonian(x.astype(np.float32).astype(np.float64), 0.1)) < 2e-6
+
+
+def test_uniform_fill_matches_host_hash(gpu):
+    from sparc_amd.workloads import Grid, uniform_block
+    g = Grid(10, 9, 8, 0.5, 0.5, 0.5)
+    with gpu.Context(g) as ctx:
+        b = ctx.block(5, gpu.F64).fill_uniform(1234, -0.5, 0.5)
+        d = b.download()
+        b32 = ctx.block(5, gpu.F32).fill_uniform(1234, -0.5, 0.5)
+        d32 = b32.download()
+    h = uniform_block(g.nd, 5, 1234, -0.5, 0.5)
+    assert np.array_equal(d, h)
+    assert np.array_equal(d32, h.astype(np.float32).astype(np.float64))
