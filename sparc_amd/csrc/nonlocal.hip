@@ -1,178 +1,399 @@
 // Kleinman-Bylander nonlocal apply  out += scale * sum_J Chi_J Gamma_J (dV Chi_J' x)
-// (SPARC Vnl_vec_mult, src/nlocVecRoutines.c:798-883), in three launches:
+// (SPARC Vnl_vec_mult, src/nlocVecRoutines.c:798-883).
 //
-//  1. nl_project: per influence entry and column group, alpha_part =
-//     Chi_J' x[grid_pos] with one wavefront per CPW columns; lanes stride the
-//     entry's points (coalesced gathers), each lane keeps CPW x NP partial sums
-//     in registers, one cross-lane reduction at the end.  No atomics.
-//  2. nl_reduce_scale: alpha_s[row] = scale * dV * Gamma[row] * sum of the
-//     partial rows of every image of the row's atom (fixed order).
-//  3. nl_apply: point-centric scatter-free apply over the covered grid points:
-//     out[q] += sum_{(J,i) at q} sum_p Chi_J[i,p] alpha_s[J,p], CBP columns per
-//     thread, so overlapping projector spheres need no atomics either.
+// The reference gathers x at each atom's sphere points and calls two small
+// dgemms per atom.  Here the projectors are re-laid out once per geometry in
+// bricks of kBX x kBY x kBZ grid points (sgpu_set_nonlocal): for each brick a
+// dense Phi_b (kBP points x np_b projector columns of every atom touching
+// it).  The apply is then three launches over bricks x column blocks:
+//
+//  1. nlb_project: part[brick rows][cols] = Phi_b' X_b.  The brick's slice of
+//     NC columns is staged in LDS (coalesced 64-byte x-rows), each wavefront
+//     takes a quarter of the brick's points and a 64-row x NC tile of
+//     accumulators, then the four partial tiles are summed through LDS.
+//  2. nlb_reduce: alpha_s[r][c] = scale dV Gamma_r sum over the brick rows of
+//     projector r (fixed order: deterministic, no atomics).
+//  3. nlb_apply: out_b += Phi_b beta_b with beta_b (np_b x NC) gathered from
+//     alpha_s into LDS; one thread per point, NC accumulators in registers.
+//
+// Bricks touch each x value once per launch, so the gather redundancy of the
+// reference (overlapping spheres read the same points once per atom) turns
+// into dense zeros inside Phi_b; both are counted in the measured rates.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
 #include "sgpu_internal.h"
 
 namespace {
 
-constexpr int kNlThreads = 256;
-constexpr int kNlWaves = kNlThreads / 64;
+template <typename T>
+struct NlbCfg;
+template <>
+struct NlbCfg<float> {
+    static constexpr int CT = 4;   // project: columns per lane
+    static constexpr int NCA = 32; // apply: columns per block
+};
+template <>
+struct NlbCfg<double> {
+    static constexpr int CT = 2;
+    static constexpr int NCA = 16;
+};
 
-template <typename T, int NP>
-__global__ __launch_bounds__(kNlThreads) void nl_project(
-    const T *__restrict__ x, long ldx, int ncol, const int *__restrict__ ndc,
-    const int *__restrict__ nproj, const long *__restrict__ pos_off,
-    const int *__restrict__ grid_pos, const long *__restrict__ chi_off, const T *__restrict__ chi,
-    const int *__restrict__ part_row, int n_part, T *__restrict__ part)
+template <typename T>
+struct vec4;
+template <>
+struct vec4<float> {
+    using type = float4;
+};
+template <>
+struct vec4<double> {
+    using type = double4;
+};
+
+constexpr int kWaves = 4;
+constexpr int kJT = 4;   // projector rows per lane (one vec4 load)
+constexpr int kJG = 16;  // lanes along projector rows -> 64 rows per chunk
+constexpr int kCG = 4;   // lanes along columns
+
+__device__ __forceinline__ long brick_point(int q, int x0, int y0, int z0, int nx, int ny, int nz)
 {
-    constexpr int CPW = NP >= 64 ? 1 : 64 / NP;
-    const int a = blockIdx.x;
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int col0 = (blockIdx.y * kNlWaves + wave) * CPW;
-    if (col0 >= ncol) return;
-    const int nd_a = ndc[a], np = nproj[a];
-    const int *pos = grid_pos + pos_off[a];
-    const T *ch = chi + chi_off[a];
-    T acc[CPW][NP];
+    const int qx = q % kBX, qy = (q / kBX) % kBY, qz = q / (kBX * kBY);
+    const int gx = x0 + qx, gy = y0 + qy, gz = z0 + qz;
+    if (gx >= nx || gy >= ny || gz >= nz) return -1;
+    return ((long)gz * ny + gy) * nx + gx;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void nlb_project(const T *__restrict__ X, long ldx, int ncol,
+                                                   int nx, int ny, int nz,
+                                                   const int *__restrict__ brick_org,
+                                                   const int *__restrict__ brick_np,
+                                                   const long *__restrict__ brick_row,
+                                                   const T *__restrict__ phi, T *__restrict__ part)
+{
+    constexpr int CT = NlbCfg<T>::CT;
+    constexpr int NC = kCG * CT;
+    constexpr int LDC = NC + 1;
+    __shared__ T xs[kBP * LDC];
+    __shared__ T red[kWaves][64][kJT * CT];
+    const int b = blockIdx.y;
+    const int col0 = blockIdx.x * NC;
+    const int x0 = brick_org[3 * b], y0 = brick_org[3 * b + 1], z0 = brick_org[3 * b + 2];
+    for (int e = threadIdx.x; e < kBP * NC; e += 256) {
+        const int c = e / kBP, q = e - c * kBP;
+        const long pt = brick_point(q, x0, y0, z0, nx, ny, nz);
+        xs[q * LDC + c] = (pt >= 0 && col0 + c < ncol) ? X[(long)(col0 + c) * ldx + pt] : T(0);
+    }
+    __syncthreads();
+    const int np = brick_np[b];
+    const long row0 = brick_row[b];
+    const T *ph = phi + row0 * kBP;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int jg = lane % kJG, cg = lane / kJG;
+    using V4 = typename vec4<T>::type;
+    for (int j0 = 0; j0 < np; j0 += kJG * kJT) {
+        const int jb = j0 + jg * kJT;
+        T acc[kJT][CT];
 #pragma unroll
-    for (int c = 0; c < CPW; c++)
+        for (int a = 0; a < kJT; a++)
 #pragma unroll
-        for (int p = 0; p < NP; p++) acc[c][p] = T(0);
-    for (int i = lane; i < nd_a; i += 64) {
-        const long q = pos[i];
-        T xv[CPW];
+            for (int c = 0; c < CT; c++) acc[a][c] = T(0);
+        if (jb < np) {
+            const int qs = wave * (kBP / kWaves);
+            for (int q = qs; q < qs + kBP / kWaves; q++) {
+                const V4 p4 = *reinterpret_cast<const V4 *>(ph + (long)q * np + jb);
+                const T pv[kJT] = {p4.x, p4.y, p4.z, p4.w};
 #pragma unroll
-        for (int c = 0; c < CPW; c++)
-            xv[c] = (col0 + c < ncol) ? x[(long)(col0 + c) * ldx + q] : T(0);
+                for (int c = 0; c < CT; c++) {
+                    const T xv = xs[q * LDC + cg * CT + c];
 #pragma unroll
-        for (int p = 0; p < NP; p++) {
-            if (p < np) {
-                const T cv = ch[(long)p * nd_a + i];
-#pragma unroll
-                for (int c = 0; c < CPW; c++) acc[c][p] += cv * xv[c];
+                    for (int a = 0; a < kJT; a++) acc[a][c] += pv[a] * xv;
+                }
             }
         }
-    }
-    const int prow = part_row[a];
 #pragma unroll
-    for (int c = 0; c < CPW; c++) {
+        for (int a = 0; a < kJT; a++)
 #pragma unroll
-        for (int p = 0; p < NP; p++) {
-            if (p >= np) continue;
-            T s = acc[c][p];
-            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-            if (lane == 0 && col0 + c < ncol) part[(long)(col0 + c) * n_part + prow + p] = s;
+            for (int c = 0; c < CT; c++) red[wave][lane][a * CT + c] = acc[a][c];
+        __syncthreads();
+        // 64 lanes x kJT*CT values; 256 threads sum the four wave partials
+        for (int e = threadIdx.x; e < 64 * kJT * CT; e += 256) {
+            const int l = e / (kJT * CT), k = e - l * (kJT * CT);
+            const int a = k / CT, c = k - a * CT;
+            const int j = j0 + (l % kJG) * kJT + a;
+            const int col = col0 + (l / kJG) * CT + c;
+            if (j < np && col < ncol) {
+                const T s = red[0][l][k] + red[1][l][k] + red[2][l][k] + red[3][l][k];
+                part[(row0 + j) * ncol + col] = s;
+            }
         }
+        __syncthreads();
     }
 }
 
 template <typename T>
-__global__ void nl_reduce_scale(const T *__restrict__ part, int n_part, int n_alpha, int ncol,
-                                const int *__restrict__ arow_ptr, const int *__restrict__ arow_src,
-                                const double *__restrict__ gamma, double scale,
-                                T *__restrict__ alpha_s)
+__global__ void nlb_reduce(const T *__restrict__ part, int ncol, int n_alpha,
+                           const int *__restrict__ arow_ptr, const int *__restrict__ arow_src,
+                           const double *__restrict__ gamma, double scale, T *__restrict__ alpha_s)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)n_alpha * ncol) return;
-    const int col = (int)(t / n_alpha);
-    const int r = (int)(t - (long)col * n_alpha);
+    const int r = (int)(t / ncol);
+    const int col = (int)(t - (long)r * ncol);
     T s = T(0);
-    for (int e = arow_ptr[r]; e < arow_ptr[r + 1]; e++) s += part[(long)col * n_part + arow_src[e]];
-    alpha_s[(long)col * n_alpha + r] = (T)(scale * gamma[r]) * s;
+    for (int e = arow_ptr[r]; e < arow_ptr[r + 1]; e++) s += part[(long)arow_src[e] * ncol + col];
+    alpha_s[(long)r * ncol + col] = (T)(scale * gamma[r]) * s;
 }
-
-constexpr int kCBP = 8;
 
 template <typename T>
-__global__ __launch_bounds__(256) void nl_apply(
-    T *__restrict__ out, long ldo, int ncol, const int *__restrict__ cov_pts, int n_cov,
-    const int *__restrict__ inv_ptr, const int *__restrict__ inv_ent,
-    const int *__restrict__ inv_loc, const int *__restrict__ ndc, const int *__restrict__ nproj,
-    const long *__restrict__ chi_off, const int *__restrict__ alpha_off,
-    const T *__restrict__ chi, const T *__restrict__ alpha_s, int n_alpha)
+__global__ __launch_bounds__(256) void nlb_apply(T *__restrict__ out, long ldo, int ncol, int nx,
+                                                 int ny, int nz, const int *__restrict__ brick_org,
+                                                 const int *__restrict__ brick_np,
+                                                 const long *__restrict__ brick_row,
+                                                 const int *__restrict__ row_alpha,
+                                                 const T *__restrict__ phi,
+                                                 const T *__restrict__ alpha_s, int max_np)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_cov) return;
-    const int col0 = blockIdx.y * kCBP;
-    const int q = cov_pts[t];
-    T acc[kCBP];
-#pragma unroll
-    for (int c = 0; c < kCBP; c++) acc[c] = T(0);
-    for (int e = inv_ptr[q]; e < inv_ptr[q + 1]; e++) {
-        const int a = inv_ent[e], i = inv_loc[e];
-        const int np = nproj[a], nd_a = ndc[a];
-        const T *ch = chi + chi_off[a] + i;
-        const T *al = alpha_s + alpha_off[a];
-        for (int p = 0; p < np; p++) {
-            const T cv = ch[(long)p * nd_a];
-#pragma unroll
-            for (int c = 0; c < kCBP; c++)
-                if (col0 + c < ncol) acc[c] += cv * al[(long)(col0 + c) * n_alpha + p];
-        }
+    constexpr int NC = NlbCfg<T>::NCA;
+    extern __shared__ unsigned char smem_raw[];
+    T *beta = reinterpret_cast<T *>(smem_raw);  // np x NC
+    const int b = blockIdx.y;
+    const int col0 = blockIdx.x * NC;
+    const int x0 = brick_org[3 * b], y0 = brick_org[3 * b + 1], z0 = brick_org[3 * b + 2];
+    const int np = brick_np[b];
+    const long row0 = brick_row[b];
+    for (int e = threadIdx.x; e < np * NC; e += 256) {
+        const int j = e / NC, c = e - j * NC;
+        const int r = row_alpha[row0 + j];
+        beta[e] = (r >= 0 && col0 + c < ncol) ? alpha_s[(long)r * ncol + col0 + c] : T(0);
     }
+    __syncthreads();
+    const T *ph = phi + row0 * kBP;
+    using V4 = typename vec4<T>::type;
+    for (int q = threadIdx.x; q < kBP; q += 256) {
+        const long pt = brick_point(q, x0, y0, z0, nx, ny, nz);
+        if (pt < 0) continue;
+        T acc[NC];
 #pragma unroll
-    for (int c = 0; c < kCBP; c++)
-        if (col0 + c < ncol) out[(long)(col0 + c) * ldo + q] += acc[c];
-}
-
-template <typename T, int NP>
-void launch_project(sgpu_context *ctx, const T *x, long ldx, int ncol, const T *chi, T *part)
-{
-    constexpr int CPW = NP >= 64 ? 1 : 64 / NP;
-    const sgpu_nonlocal &nl = ctx->nl;
-    dim3 grid(nl.n_infl, (ncol + kNlWaves * CPW - 1) / (kNlWaves * CPW));
-    nl_project<T, NP><<<grid, kNlThreads, 0, ctx->stream>>>(
-        x, ldx, ncol, nl.ndc, nl.nproj, nl.pos_off, nl.grid_pos, nl.chi_off, chi, nl.part_row,
-        nl.n_part, part);
+        for (int c = 0; c < NC; c++) acc[c] = T(0);
+        const T *prow = ph + (long)q * np;
+        for (int j = 0; j < np; j += 4) {
+            const V4 p4 = *reinterpret_cast<const V4 *>(prow + j);
+            const T pv[4] = {p4.x, p4.y, p4.z, p4.w};
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                const T *bj = beta + (j + a) * NC;
+#pragma unroll
+                for (int c = 0; c < NC; c++) acc[c] += pv[a] * bj[c];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+            if (col0 + c < ncol) out[(long)(col0 + c) * ldo + pt] += acc[c];
+    }
 }
 
 template <typename T>
 int launch_vnl_typed(sgpu_context *ctx, int ncol, const T *x, long ldx, T *out, long ldo,
-                     double scale, const T *chi)
+                     double scale, const T *phi)
 {
     const sgpu_nonlocal &nl = ctx->nl;
-    if (nl.n_infl == 0 || nl.n_alpha == 0 || ncol <= 0) return SGPU_OK;
-    if (ncol > 65535 * kCBP) {
-        sgpu_set_error("nonlocal: ncol %d exceeds the launch limit", ncol);
+    if (nl.n_bricks == 0 || nl.n_alpha == 0 || ncol <= 0) return SGPU_OK;
+    if (nl.n_bricks > 65535) {
+        sgpu_set_error("nonlocal: %d bricks exceed the launch limit", nl.n_bricks);
         return SGPU_ERR_ARG;
     }
     void *buf = nullptr;
-    const size_t part_elems = (size_t)nl.n_part * ncol;
+    const size_t part_elems = (size_t)nl.n_rows * ncol;
     const size_t alpha_elems = (size_t)nl.n_alpha * ncol;
     SGPU_CHECK(sgpu_scratch(ctx, (part_elems + alpha_elems) * sizeof(T) + 256, &buf));
     T *part = (T *)buf;
     T *alpha_s = part + part_elems;
-    if (nl.max_np <= 8) launch_project<T, 8>(ctx, x, ldx, ncol, chi, part);
-    else if (nl.max_np <= 16) launch_project<T, 16>(ctx, x, ldx, ncol, chi, part);
-    else if (nl.max_np <= 32) launch_project<T, 32>(ctx, x, ldx, ncol, chi, part);
-    else launch_project<T, 64>(ctx, x, ldx, ncol, chi, part);
+    constexpr int NCP = kCG * NlbCfg<T>::CT;
+    constexpr int NCA = NlbCfg<T>::NCA;
+    sgpu_tmark t;
+    SGPU_CHECK(sgpu_tbegin(ctx, &t));
+    nlb_project<T><<<dim3((ncol + NCP - 1) / NCP, nl.n_bricks), 256, 0, ctx->stream>>>(
+        x, ldx, ncol, ctx->nx, ctx->ny, ctx->nz, nl.brick_org, nl.brick_np, nl.brick_row, phi, part);
     SGPU_HIP(hipGetLastError());
-    {
-        long total = (long)nl.n_alpha * ncol;
-        nl_reduce_scale<T><<<(unsigned)((total + 255) / 256), 256, 0, ctx->stream>>>(
-            part, nl.n_part, nl.n_alpha, ncol, nl.arow_ptr, nl.arow_src, nl.gamma,
-            scale * nl.dV, alpha_s);
-        SGPU_HIP(hipGetLastError());
-    }
-    if (nl.n_cov > 0) {
-        dim3 grid((nl.n_cov + 255) / 256, (ncol + kCBP - 1) / kCBP);
-        nl_apply<T><<<grid, 256, 0, ctx->stream>>>(out, ldo, ncol, nl.cov_pts, nl.n_cov,
-                                                    nl.inv_ptr, nl.inv_ent, nl.inv_loc, nl.ndc,
-                                                    nl.nproj, nl.chi_off, nl.alpha_off, chi,
-                                                    alpha_s, nl.n_alpha);
-        SGPU_HIP(hipGetLastError());
-    }
+    SGPU_CHECK(sgpu_tend(ctx, &t, "nl_project"));
+    const long total = (long)nl.n_alpha * ncol;
+    nlb_reduce<T><<<(unsigned)((total + 255) / 256), 256, 0, ctx->stream>>>(
+        part, ncol, nl.n_alpha, nl.arow_ptr, nl.arow_src, nl.gamma, scale * nl.dV, alpha_s);
+    SGPU_HIP(hipGetLastError());
+    SGPU_CHECK(sgpu_tbegin(ctx, &t));
+    const size_t shmem = (size_t)nl.max_np * NCA * sizeof(T);
+    nlb_apply<T><<<dim3((ncol + NCA - 1) / NCA, nl.n_bricks), 256, shmem, ctx->stream>>>(
+        out, ldo, ncol, ctx->nx, ctx->ny, ctx->nz, nl.brick_org, nl.brick_np, nl.brick_row,
+        nl.row_alpha, phi, alpha_s, nl.max_np);
+    SGPU_HIP(hipGetLastError());
+    SGPU_CHECK(sgpu_tend(ctx, &t, "nl_apply"));
+    return SGPU_OK;
+}
+
+template <typename T>
+int upload(T **dst, const T *src, size_t n, hipStream_t s)
+{
+    SGPU_HIP(hipMalloc((void **)dst, (n ? n : 1) * sizeof(T)));
+    if (n) SGPU_HIP(hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
     return SGPU_OK;
 }
 
 }  // namespace
+
+void sgpu_free_nonlocal(sgpu_nonlocal &nl)
+{
+    void *ptrs[] = {nl.brick_org, nl.brick_np, nl.brick_row, nl.row_alpha, nl.phi64,
+                    nl.phi32,     nl.arow_ptr, nl.arow_src,  nl.gamma};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    nl = sgpu_nonlocal();
+}
+
+extern "C" int sgpu_set_nonlocal(sgpu_context *ctx, int n_influence, const int *ndc,
+                                 const int *nproj, const long *pos_off, const int *grid_pos,
+                                 const long *chi_off, const double *chi, const int *alpha_off,
+                                 int n_alpha, const double *gamma, double dV)
+{
+    if (!ctx || n_influence < 0 || n_alpha < 0) {
+        sgpu_set_error("sgpu_set_nonlocal: bad arguments");
+        return SGPU_ERR_ARG;
+    }
+    SGPU_HIP(hipSetDevice(ctx->device));
+    SGPU_HIP(hipStreamSynchronize(ctx->stream));
+    sgpu_free_nonlocal(ctx->nl);
+    if (n_influence == 0 || n_alpha == 0) return SGPU_OK;
+    if (!ndc || !nproj || !pos_off || !grid_pos || !chi_off || !chi || !alpha_off || !gamma) {
+        sgpu_set_error("sgpu_set_nonlocal: null array");
+        return SGPU_ERR_ARG;
+    }
+    const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
+    const int nbx = (nx + kBX - 1) / kBX, nby = (ny + kBY - 1) / kBY, nbz = (nz + kBZ - 1) / kBZ;
+    // (brick, entry) pairs, validated
+    std::vector<std::pair<int, int>> pairs;
+    for (int a = 0; a < n_influence; a++) {
+        if (ndc[a] < 0 || nproj[a] < 0 || alpha_off[a] < 0 || alpha_off[a] + nproj[a] > n_alpha ||
+            pos_off[a] < 0 || chi_off[a] < 0) {
+            sgpu_set_error("sgpu_set_nonlocal: entry %d out of range", a);
+            return SGPU_ERR_ARG;
+        }
+        std::vector<int> bl;
+        for (int i = 0; i < ndc[a]; i++) {
+            const int q = grid_pos[pos_off[a] + i];
+            if (q < 0 || q >= ctx->nd) {
+                sgpu_set_error("sgpu_set_nonlocal: grid_pos %d of entry %d outside the grid", q, a);
+                return SGPU_ERR_ARG;
+            }
+            const int ix = q % nx, iy = (q / nx) % ny, iz = q / (nx * ny);
+            bl.push_back((iz / kBZ * nby + iy / kBY) * nbx + ix / kBX);
+        }
+        std::sort(bl.begin(), bl.end());
+        bl.erase(std::unique(bl.begin(), bl.end()), bl.end());
+        for (int b : bl) pairs.push_back({b, a});
+    }
+    std::sort(pairs.begin(), pairs.end());
+    // bricks, their column layout and row offsets
+    std::vector<int> bid, borg, bnp;
+    std::vector<long> brow;
+    std::vector<int> row_alpha;
+    // column offset of entry a inside brick k: (k, a) -> col, looked up by pair index
+    std::vector<int> pair_col(pairs.size());
+    std::vector<int> pair_brick(pairs.size());
+    long nrows = 0;
+    int max_np = 0;
+    for (size_t s = 0; s < pairs.size();) {
+        size_t e = s;
+        int cols = 0;
+        while (e < pairs.size() && pairs[e].first == pairs[s].first) {
+            pair_col[e] = cols;
+            pair_brick[e] = (int)bid.size();
+            cols += nproj[pairs[e].second];
+            e++;
+        }
+        const int padded = (cols + 3) / 4 * 4;
+        const int b = pairs[s].first;
+        bid.push_back(b);
+        borg.push_back((b % nbx) * kBX);
+        borg.push_back(((b / nbx) % nby) * kBY);
+        borg.push_back((b / (nbx * nby)) * kBZ);
+        bnp.push_back(padded);
+        brow.push_back(nrows);
+        for (size_t k = s; k < e; k++)
+            for (int p = 0; p < nproj[pairs[k].second]; p++)
+                row_alpha.push_back(alpha_off[pairs[k].second] + p);
+        for (int p = cols; p < padded; p++) row_alpha.push_back(-1);
+        nrows += padded;
+        max_np = std::max(max_np, padded);
+        s = e;
+    }
+    const int nb = (int)bid.size();
+    if ((size_t)max_np * 32 * sizeof(double) > 60 * 1024) {
+        sgpu_set_error("sgpu_set_nonlocal: %d projector columns in one brick exceed LDS staging", max_np);
+        return SGPU_ERR_ARG;
+    }
+    // dense Phi per brick
+    std::vector<double> phi((size_t)nrows * kBP, 0.0);
+    {
+        // pair index lookup per entry: entries' bricks in ascending order
+        std::vector<std::vector<std::pair<int, int>>> ent_pairs(n_influence);
+        for (size_t k = 0; k < pairs.size(); k++)
+            ent_pairs[pairs[k].second].push_back({pairs[k].first, (int)k});
+        for (int a = 0; a < n_influence; a++) {
+            auto &ep = ent_pairs[a];
+            for (int i = 0; i < ndc[a]; i++) {
+                const int q = grid_pos[pos_off[a] + i];
+                const int ix = q % nx, iy = (q / nx) % ny, iz = q / (nx * ny);
+                const int b = (iz / kBZ * nby + iy / kBY) * nbx + ix / kBX;
+                auto it = std::lower_bound(ep.begin(), ep.end(), std::make_pair(b, -1));
+                const int k = it->second;
+                const int kb = pair_brick[k];
+                const int ql = (ix % kBX) + kBX * ((iy % kBY) + kBY * (iz % kBZ));
+                const long base = (brow[kb] * kBP) + (long)ql * bnp[kb] + pair_col[k];
+                for (int p = 0; p < nproj[a]; p++)
+                    phi[base + p] += chi[chi_off[a] + (long)p * ndc[a] + i];
+            }
+        }
+    }
+    // alpha row -> brick rows
+    std::vector<int> arow_ptr(n_alpha + 1, 0), arow_src;
+    for (long r = 0; r < nrows; r++)
+        if (row_alpha[r] >= 0) arow_ptr[row_alpha[r] + 1]++;
+    for (int r = 0; r < n_alpha; r++) arow_ptr[r + 1] += arow_ptr[r];
+    arow_src.resize(std::max(1, arow_ptr[n_alpha]));
+    {
+        std::vector<int> fill(arow_ptr.begin(), arow_ptr.end() - 1);
+        for (long r = 0; r < nrows; r++)
+            if (row_alpha[r] >= 0) arow_src[fill[row_alpha[r]]++] = (int)r;
+    }
+    sgpu_nonlocal &nl = ctx->nl;
+    hipStream_t s = ctx->stream;
+    nl.n_infl = n_influence;
+    nl.n_alpha = n_alpha;
+    nl.n_bricks = nb;
+    nl.n_rows = nrows;
+    nl.max_np = max_np;
+    nl.dV = dV;
+    SGPU_CHECK(upload(&nl.brick_org, borg.data(), borg.size(), s));
+    SGPU_CHECK(upload(&nl.brick_np, bnp.data(), bnp.size(), s));
+    SGPU_CHECK(upload(&nl.brick_row, brow.data(), brow.size(), s));
+    SGPU_CHECK(upload(&nl.row_alpha, row_alpha.data(), row_alpha.size(), s));
+    SGPU_CHECK(upload(&nl.phi64, phi.data(), phi.size(), s));
+    SGPU_HIP(hipMalloc((void **)&nl.phi32, std::max<size_t>(phi.size(), 1) * sizeof(float)));
+    SGPU_CHECK(sgpu_convert(ctx, nl.phi64, SGPU_F64, nl.phi32, SGPU_F32, (long)phi.size()));
+    SGPU_CHECK(upload(&nl.arow_ptr, arow_ptr.data(), arow_ptr.size(), s));
+    SGPU_CHECK(upload(&nl.arow_src, arow_src.data(), arow_src.size(), s));
+    SGPU_CHECK(upload(&nl.gamma, gamma, (size_t)n_alpha, s));
+    SGPU_HIP(hipStreamSynchronize(s));
+    return SGPU_OK;
+}
 
 int sgpu_launch_vnl(sgpu_context *ctx, int dtype, int ncol, const void *x, long ldx, void *out,
                     long ldo, double scale)
 {
     if (dtype == SGPU_F64)
         return launch_vnl_typed<double>(ctx, ncol, (const double *)x, ldx, (double *)out, ldo,
-                                        scale, ctx->nl.chi64);
+                                        scale, ctx->nl.phi64);
     return launch_vnl_typed<float>(ctx, ncol, (const float *)x, ldx, (float *)out, ldo, scale,
-                                   ctx->nl.chi32);
+                                   ctx->nl.phi32);
 }

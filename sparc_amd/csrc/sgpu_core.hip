@@ -106,16 +106,6 @@ extern "C" int sgpu_create(sgpu_context **out, int device, int nx, int ny, int n
     return SGPU_OK;
 }
 
-static void free_nonlocal(sgpu_nonlocal &nl)
-{
-    void *ptrs[] = {nl.ndc, nl.nproj, nl.alpha_off, nl.pos_off, nl.chi_off, nl.grid_pos,
-                    nl.chi64, nl.chi32, nl.gamma, nl.inv_ptr, nl.inv_ent, nl.inv_loc,
-                    nl.cov_pts, nl.part_row, nl.arow_ptr, nl.arow_src};
-    for (void *p : ptrs)
-        if (p) (void)hipFree(p);
-    nl = sgpu_nonlocal();
-}
-
 extern "C" int sgpu_destroy(sgpu_context *ctx)
 {
     if (!ctx) return SGPU_OK;
@@ -127,7 +117,7 @@ extern "C" int sgpu_destroy(sgpu_context *ctx)
         b->ctx = nullptr;  // orphaned: sgpu_block_destroy only frees the handle
     }
     ctx->blocks.clear();
-    free_nonlocal(ctx->nl);
+    sgpu_free_nonlocal(ctx->nl);
     if (ctx->veff64) (void)hipFree(ctx->veff64);
     if (ctx->veff32) (void)hipFree(ctx->veff32);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
@@ -282,114 +272,6 @@ extern "C" int sgpu_set_veff(sgpu_context *ctx, const double *veff)
     SGPU_CHECK(sgpu_convert(ctx, ctx->veff64, SGPU_F64, ctx->veff32, SGPU_F32, ctx->nd));
     SGPU_HIP(hipStreamSynchronize(ctx->stream));
     ctx->has_veff = true;
-    return SGPU_OK;
-}
-
-template <typename T>
-static int upload(T **dst, const T *src, size_t n, hipStream_t s)
-{
-    SGPU_HIP(hipMalloc((void **)dst, (n ? n : 1) * sizeof(T)));
-    if (n) SGPU_HIP(hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
-    return SGPU_OK;
-}
-
-extern "C" int sgpu_set_nonlocal(sgpu_context *ctx, int n_influence, const int *ndc,
-                                 const int *nproj, const long *pos_off, const int *grid_pos,
-                                 const long *chi_off, const double *chi, const int *alpha_off,
-                                 int n_alpha, const double *gamma, double dV)
-{
-    if (!ctx || n_influence < 0 || n_alpha < 0) {
-        sgpu_set_error("sgpu_set_nonlocal: bad arguments");
-        return SGPU_ERR_ARG;
-    }
-    SGPU_HIP(hipSetDevice(ctx->device));
-    SGPU_HIP(hipStreamSynchronize(ctx->stream));
-    free_nonlocal(ctx->nl);
-    if (n_influence == 0 || n_alpha == 0) return SGPU_OK;
-    if (!ndc || !nproj || !pos_off || !grid_pos || !chi_off || !chi || !alpha_off || !gamma) {
-        sgpu_set_error("sgpu_set_nonlocal: null array");
-        return SGPU_ERR_ARG;
-    }
-    // validate and size the flattened arrays on the host before any upload
-    long npts = 0, nchi = 0;
-    int max_np = 0;
-    for (int a = 0; a < n_influence; a++) {
-        if (ndc[a] < 0 || nproj[a] < 0 || nproj[a] > 64 || alpha_off[a] < 0 ||
-            alpha_off[a] + nproj[a] > n_alpha || pos_off[a] < 0 || chi_off[a] < 0) {
-            sgpu_set_error("sgpu_set_nonlocal: entry %d out of range", a);
-            return SGPU_ERR_ARG;
-        }
-        npts = std::max(npts, pos_off[a] + ndc[a]);
-        nchi = std::max(nchi, chi_off[a] + (long)ndc[a] * nproj[a]);
-        max_np = std::max(max_np, nproj[a]);
-    }
-    for (long i = 0; i < npts; i++)
-        if (grid_pos[i] < 0 || grid_pos[i] >= ctx->nd) {
-            sgpu_set_error("sgpu_set_nonlocal: grid_pos[%ld]=%d outside the grid", i, grid_pos[i]);
-            return SGPU_ERR_ARG;
-        }
-    // point-centric inverse map (counting sort by grid point)
-    std::vector<int> cnt(ctx->nd + 1, 0);
-    long nent = 0;
-    for (int a = 0; a < n_influence; a++)
-        for (int i = 0; i < ndc[a]; i++) { cnt[grid_pos[pos_off[a] + i] + 1]++; nent++; }
-    if (nent > (1L << 31) - 1) {
-        sgpu_set_error("sgpu_set_nonlocal: too many projector points");
-        return SGPU_ERR_ARG;
-    }
-    for (long q = 0; q < ctx->nd; q++) cnt[q + 1] += cnt[q];
-    std::vector<int> inv_ent(nent > 0 ? nent : 1), inv_loc(nent > 0 ? nent : 1);
-    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
-    for (int a = 0; a < n_influence; a++)
-        for (int i = 0; i < ndc[a]; i++) {
-            int q = grid_pos[pos_off[a] + i];
-            int slot = fill[q]++;
-            inv_ent[slot] = a;
-            inv_loc[slot] = i;
-        }
-    std::vector<int> cov;
-    for (long q = 0; q < ctx->nd; q++)
-        if (cnt[q + 1] > cnt[q]) cov.push_back((int)q);
-    // partial rows per entry and the alpha-row -> partial-row lists
-    std::vector<int> part_row(n_influence);
-    int n_part = 0;
-    for (int a = 0; a < n_influence; a++) { part_row[a] = n_part; n_part += nproj[a]; }
-    std::vector<int> arow_ptr(n_alpha + 1, 0), arow_src(n_part > 0 ? n_part : 1);
-    for (int a = 0; a < n_influence; a++)
-        for (int p = 0; p < nproj[a]; p++) arow_ptr[alpha_off[a] + p + 1]++;
-    for (int r = 0; r < n_alpha; r++) arow_ptr[r + 1] += arow_ptr[r];
-    {
-        std::vector<int> af(arow_ptr.begin(), arow_ptr.end() - 1);
-        for (int a = 0; a < n_influence; a++)
-            for (int p = 0; p < nproj[a]; p++) arow_src[af[alpha_off[a] + p]++] = part_row[a] + p;
-    }
-    sgpu_nonlocal &nl = ctx->nl;
-    nl.n_infl = n_influence;
-    nl.n_alpha = n_alpha;
-    nl.max_np = max_np;
-    nl.total_pts = npts;
-    nl.dV = dV;
-    hipStream_t s = ctx->stream;
-    SGPU_CHECK(upload(&nl.ndc, ndc, n_influence, s));
-    SGPU_CHECK(upload(&nl.nproj, nproj, n_influence, s));
-    SGPU_CHECK(upload(&nl.alpha_off, alpha_off, n_influence, s));
-    SGPU_CHECK(upload(&nl.pos_off, pos_off, n_influence, s));
-    SGPU_CHECK(upload(&nl.chi_off, chi_off, n_influence, s));
-    SGPU_CHECK(upload(&nl.grid_pos, grid_pos, npts, s));
-    SGPU_CHECK(upload(&nl.chi64, chi, nchi, s));
-    SGPU_HIP(hipMalloc((void **)&nl.chi32, (nchi ? nchi : 1) * sizeof(float)));
-    SGPU_CHECK(sgpu_convert(ctx, nl.chi64, SGPU_F64, nl.chi32, SGPU_F32, nchi));
-    SGPU_CHECK(upload(&nl.gamma, gamma, n_alpha, s));
-    SGPU_CHECK(upload(&nl.inv_ptr, cnt.data(), ctx->nd + 1, s));
-    SGPU_CHECK(upload(&nl.inv_ent, inv_ent.data(), (size_t)nent, s));
-    SGPU_CHECK(upload(&nl.inv_loc, inv_loc.data(), (size_t)nent, s));
-    nl.n_cov = (int)cov.size();
-    SGPU_CHECK(upload(&nl.cov_pts, cov.data(), cov.size(), s));
-    nl.n_part = n_part;
-    SGPU_CHECK(upload(&nl.part_row, part_row.data(), (size_t)n_influence, s));
-    SGPU_CHECK(upload(&nl.arow_ptr, arow_ptr.data(), (size_t)n_alpha + 1, s));
-    SGPU_CHECK(upload(&nl.arow_src, arow_src.data(), (size_t)n_part, s));
-    SGPU_HIP(hipStreamSynchronize(s));
     return SGPU_OK;
 }
 

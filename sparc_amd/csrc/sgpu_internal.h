@@ -50,32 +50,30 @@ struct sgpu_block {
     void *ptr;   // device storage, ncol * ld elements
 };
 
-// Device-side nonlocal projector set (see sgpu_set_nonlocal).
+// Nonlocal projectors re-laid out in bricks of kBX x kBY x kBZ grid points.
+// For every brick touched by at least one projector sphere, Phi_b is a dense
+// kBP x np_b row-major block (np_b = projector columns of all influence
+// entries touching the brick, padded to a multiple of 4 with zero columns);
+// brick row j maps to projector (alpha) row row_alpha[brick_row[b] + j]
+// (-1 for padding).  Periodic images of one atom are separate entries whose
+// columns map to the same alpha rows.
+constexpr int kBX = 16, kBY = 8, kBZ = 4, kBP = kBX * kBY * kBZ;
+
 struct sgpu_nonlocal {
     int n_infl = 0;
     int n_alpha = 0;
+    int n_bricks = 0;
+    long n_rows = 0;            // sum of padded brick rows
     int max_np = 0;
-    long total_pts = 0;
-    int *ndc = nullptr, *nproj = nullptr, *alpha_off = nullptr;
-    long *pos_off = nullptr, *chi_off = nullptr;
-    int *grid_pos = nullptr;
-    double *chi64 = nullptr;
-    float *chi32 = nullptr;
-    double *gamma = nullptr;
-    // point-centric inverse map: for grid point q, entries inv_ptr[q]..inv_ptr[q+1]-1
-    // give (influence entry, local index within that entry's ndc points).
-    int *inv_ptr = nullptr;
-    int *inv_ent = nullptr;
-    int *inv_loc = nullptr;
-    int *cov_pts = nullptr;   // grid points with at least one projector entry
-    int n_cov = 0;
-    // per-entry partial rows (entry a owns rows part_row[a] .. + nproj[a]) and,
-    // for each alpha row r, the partial rows arow_src[arow_ptr[r] ..] summed
-    // into it (periodic images of one atom share alpha rows)
-    int *part_row = nullptr;
-    int n_part = 0;
-    int *arow_ptr = nullptr;
+    int *brick_org = nullptr;   // x0, y0, z0 per brick
+    int *brick_np = nullptr;    // padded rows per brick
+    long *brick_row = nullptr;  // first brick row of each brick
+    int *row_alpha = nullptr;   // alpha row of each brick row (-1: padding)
+    double *phi64 = nullptr;    // Phi_b at brick_row[b] * kBP
+    float *phi32 = nullptr;
+    int *arow_ptr = nullptr;    // alpha row r sums brick rows arow_src[arow_ptr[r]..]
     int *arow_src = nullptr;
+    double *gamma = nullptr;    // Gamma per alpha row
     double dV = 0.0;
 };
 
@@ -140,6 +138,7 @@ int sgpu_launch_stencil(sgpu_context *ctx, int dtype, int ncol, double a, double
                         const void *v, const void *x, long ldx, void *out, long ldo,
                         const void *z, long ldz, double s1, double s2, bool allow_tiled);
 // ---- nonlocal.hip ----
+void sgpu_free_nonlocal(sgpu_nonlocal &nl);
 // out += scale * Vnl x  (alpha computed from x); dtype of x/out.
 int sgpu_launch_vnl(sgpu_context *ctx, int dtype, int ncol, const void *x, long ldx, void *out,
                     long ldo, double scale);
