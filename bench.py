@@ -114,6 +114,9 @@ def main(argv=None):
     ap.add_argument("--degree", type=int, default=24)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one device copy of a 512-vector FP32 block (known bytes) so "
+                         "that a profile holds a plain copy kernel to scale HBM counters by")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -135,6 +138,16 @@ def main(argv=None):
     bounds = bounds_lanczos(ctx, wl)
     print(f"[bench] bounds eigmin {bounds[0]:.6f} eigmax {bounds[1]:.6f} cutoff {bounds[2]:.6f}",
           flush=True)
+
+    if args.calib_copy:
+        src = ctx.block(ns, gpu.F32).fill_uniform(1, 0.0, 1.0)
+        dst = ctx.block(ns, gpu.F32)
+        dst.copy_from(src)
+        ctx.sync()
+        print(f"[bench] calibration copy: {ns} x {nd} FP32 = {4 * ns * nd} bytes read and written",
+              flush=True)
+        src.free()
+        dst.free()
 
     r32 = time_filter(ctx, gpu, wl, gpu.F32, args.steps, args.warmup, bounds)
     ms32 = r32["filter"][0] / max(r32["filter"][1], 1)

@@ -7,14 +7,15 @@
 // dense Phi_b (kBP points x np_b projector columns of every atom touching
 // it).  The apply is then three launches over bricks x column blocks:
 //
-//  1. nlb_project: part[brick rows][cols] = Phi_b' X_b.  The brick's slice of
-//     NC columns is staged in LDS (coalesced 64-byte x-rows), each wavefront
-//     takes a quarter of the brick's points and a 64-row x NC tile of
-//     accumulators, then the four partial tiles are summed through LDS.
+//  1. nlb_project: part[brick rows][cols] = Phi_b' X_b on 16x16x4 MFMA.  The
+//     brick's slice of NC columns is staged in LDS (coalesced 64-byte
+//     x-rows), each wavefront takes a quarter of the brick's points for a
+//     64-row x NC output tile, and the four partial tiles are summed in a
+//     fixed order through LDS.
 //  2. nlb_reduce: alpha_s[r][c] = scale dV Gamma_r sum over the brick rows of
 //     projector r (fixed order: deterministic, no atomics).
-//  3. nlb_apply: out_b += Phi_b beta_b with beta_b (np_b x NC) gathered from
-//     alpha_s into LDS; one thread per point, NC accumulators in registers.
+//  3. nlb_apply: out_b += Phi_b beta_b on 16x16x4 MFMA with beta_b (np_b x NC)
+//     gathered from alpha_s into LDS and Phi_b read column-major.
 //
 // Bricks touch each x value once per launch, so the gather redundancy of the
 // reference (overlapping spheres read the same points once per atom) turns
@@ -27,34 +28,44 @@
 
 namespace {
 
-template <typename T>
-struct NlbCfg;
-template <>
-struct NlbCfg<float> {
-    static constexpr int CT = 4;   // project: columns per lane
-    static constexpr int NCA = 32; // apply: columns per block
-};
-template <>
-struct NlbCfg<double> {
-    static constexpr int CT = 2;
-    static constexpr int NCA = 16;
-};
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
 
+// 16x16x4 MFMA per precision.  Operand layout (gfx950 matrix cores, checked
+// on the device): A (16x4) element (i,k) in lane i + 16k, B (4x16) element
+// (k,n) in lane n + 16k.  D (16x16) element (i,n) sits in lane n + 16(i/4),
+// register i%4, for f32, and in lane n + 16(i%4), register i/4, for f64;
+// drow(lk, r) gives the row i held in register r of a lane with lk = lane/16.
 template <typename T>
-struct vec4;
+struct Mfma;
 template <>
-struct vec4<float> {
-    using type = float4;
+struct Mfma<float> {
+    using acc = f32x4;
+    using vec = float4;              // 16-byte staging vector
+    static constexpr int NCP = 16;   // project: columns per block
+    static constexpr int NCA = 32;   // apply: columns per block
+    __device__ static acc run(float a, float b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    __device__ static int drow(int lk, int r) { return 4 * lk + r; }
 };
 template <>
-struct vec4<double> {
-    using type = double4;
+struct Mfma<double> {
+    using acc = f64x4;
+    using vec = double2;
+    static constexpr int NCP = 16;
+    static constexpr int NCA = 16;
+    __device__ static acc run(double a, double b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    __device__ static int drow(int lk, int r) { return lk + 4 * r; }
 };
 
 constexpr int kWaves = 4;
-constexpr int kJT = 4;   // projector rows per lane (one vec4 load)
-constexpr int kJG = 16;  // lanes along projector rows -> 64 rows per chunk
-constexpr int kCG = 4;   // lanes along columns
+constexpr int kRT = 4;            // 16-row tiles per projector chunk (64 rows)
+constexpr int kLDQ = kBP + 4;     // LDS stride of a staged column (bank-conflict free)
 
 __device__ __forceinline__ long brick_point(int q, int x0, int y0, int z0, int nx, int ny, int nz)
 {
@@ -64,6 +75,9 @@ __device__ __forceinline__ long brick_point(int q, int x0, int y0, int z0, int n
     return ((long)gz * ny + gy) * nx + gx;
 }
 
+// part[brick row j][col] = sum_q Phi_b[q][j] X[col][q]  on 16x16x4 MFMA.
+// The brick's NC columns are staged in LDS column-major; the four waves split
+// the 512 points and their partial tiles are summed in a fixed order.
 template <typename T>
 __global__ __launch_bounds__(256) void nlb_project(const T *__restrict__ X, long ldx, int ncol,
                                                    int nx, int ny, int nz,
@@ -72,63 +86,90 @@ __global__ __launch_bounds__(256) void nlb_project(const T *__restrict__ X, long
                                                    const long *__restrict__ brick_row,
                                                    const T *__restrict__ phi, T *__restrict__ part)
 {
-    constexpr int CT = NlbCfg<T>::CT;
-    constexpr int NC = kCG * CT;
-    constexpr int LDC = NC + 1;
-    __shared__ T xs[kBP * LDC];
-    __shared__ T red[kWaves][64][kJT * CT];
-    const int b = blockIdx.y;
-    const int col0 = blockIdx.x * NC;
+    using M = Mfma<T>;
+    constexpr int NC = M::NCP, NCT = NC / 16;
+    __shared__ __align__(16) T xs[NC * kLDQ];
+    __shared__ typename M::acc red[kRT * NCT][64];
+    // column blocks of one brick are consecutive logical ids -> one XCD/L2
+    const int lid = xcd_remap(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int b = lid / gridDim.x;
+    const int col0 = (lid % gridDim.x) * NC;
     const int x0 = brick_org[3 * b], y0 = brick_org[3 * b + 1], z0 = brick_org[3 * b + 2];
-    for (int e = threadIdx.x; e < kBP * NC; e += 256) {
-        const int c = e / kBP, q = e - c * kBP;
-        const long pt = brick_point(q, x0, y0, z0, nx, ny, nz);
-        xs[q * LDC + c] = (pt >= 0 && col0 + c < ncol) ? X[(long)(col0 + c) * ldx + pt] : T(0);
+    constexpr int VW = sizeof(typename M::vec) / sizeof(T);
+    if (nx % VW == 0 && ldx % VW == 0) {
+        // 16-byte loads: a vector never straddles the grid edge because
+        // x0 is a multiple of kBX and nx a multiple of VW
+#pragma unroll 4
+        for (int e = threadIdx.x; e < kBP * NC / VW; e += 256) {
+            const int c = e / (kBP / VW), q = (e - c * (kBP / VW)) * VW;
+            const long pt = brick_point(q, x0, y0, z0, nx, ny, nz);
+            typename M::vec v{};
+            if (pt >= 0 && col0 + c < ncol)
+                v = *reinterpret_cast<const typename M::vec *>(X + (long)(col0 + c) * ldx + pt);
+            *reinterpret_cast<typename M::vec *>(xs + c * kLDQ + q) = v;
+        }
+    } else {
+        for (int e = threadIdx.x; e < kBP * NC; e += 256) {
+            const int c = e / kBP, q = e - c * kBP;
+            const long pt = brick_point(q, x0, y0, z0, nx, ny, nz);
+            xs[c * kLDQ + q] = (pt >= 0 && col0 + c < ncol) ? X[(long)(col0 + c) * ldx + pt] : T(0);
+        }
     }
     __syncthreads();
     const int np = brick_np[b];
     const long row0 = brick_row[b];
     const T *ph = phi + row0 * kBP;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int jg = lane % kJG, cg = lane / kJG;
-    using V4 = typename vec4<T>::type;
-    for (int j0 = 0; j0 < np; j0 += kJG * kJT) {
-        const int jb = j0 + jg * kJT;
-        T acc[kJT][CT];
+    const int li = lane & 15, lk = lane >> 4;
+    const int qs = wave * (kBP / kWaves);
+    for (int j0 = 0; j0 < np; j0 += 16 * kRT) {
+        typename M::acc acc[kRT][NCT];
 #pragma unroll
-        for (int a = 0; a < kJT; a++)
+        for (int rt = 0; rt < kRT; rt++)
 #pragma unroll
-            for (int c = 0; c < CT; c++) acc[a][c] = T(0);
-        if (jb < np) {
-            const int qs = wave * (kBP / kWaves);
-            for (int q = qs; q < qs + kBP / kWaves; q++) {
-                const V4 p4 = *reinterpret_cast<const V4 *>(ph + (long)q * np + jb);
-                const T pv[kJT] = {p4.x, p4.y, p4.z, p4.w};
+            for (int ct = 0; ct < NCT; ct++) acc[rt][ct] = typename M::acc{0, 0, 0, 0};
+        for (int k = qs; k < qs + kBP / kWaves; k += 4) {
+            const int q = k + lk;
+            T bv[NCT];
 #pragma unroll
-                for (int c = 0; c < CT; c++) {
-                    const T xv = xs[q * LDC + cg * CT + c];
+            for (int ct = 0; ct < NCT; ct++) bv[ct] = xs[(ct * 16 + li) * kLDQ + q];
 #pragma unroll
-                    for (int a = 0; a < kJT; a++) acc[a][c] += pv[a] * xv;
+            for (int rt = 0; rt < kRT; rt++) {
+                if (j0 + rt * 16 < np) {
+                    const int j = j0 + rt * 16 + li;
+                    const T a = j < np ? ph[(long)q * np + j] : T(0);
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ct++) acc[rt][ct] = M::run(a, bv[ct], acc[rt][ct]);
                 }
             }
         }
+        // fixed-order cross-wave sum: wave 3 -> 2 -> 1 -> 0
+        for (int w = kWaves - 1; w >= 1; w--) {
+            if (wave == w)
 #pragma unroll
-        for (int a = 0; a < kJT; a++)
+                for (int rt = 0; rt < kRT; rt++)
 #pragma unroll
-            for (int c = 0; c < CT; c++) red[wave][lane][a * CT + c] = acc[a][c];
-        __syncthreads();
-        // 64 lanes x kJT*CT values; 256 threads sum the four wave partials
-        for (int e = threadIdx.x; e < 64 * kJT * CT; e += 256) {
-            const int l = e / (kJT * CT), k = e - l * (kJT * CT);
-            const int a = k / CT, c = k - a * CT;
-            const int j = j0 + (l % kJG) * kJT + a;
-            const int col = col0 + (l / kJG) * CT + c;
-            if (j < np && col < ncol) {
-                const T s = red[0][l][k] + red[1][l][k] + red[2][l][k] + red[3][l][k];
-                part[(row0 + j) * ncol + col] = s;
-            }
+                    for (int ct = 0; ct < NCT; ct++) red[rt * NCT + ct][lane] = acc[rt][ct];
+            __syncthreads();
+            if (wave == w - 1)
+#pragma unroll
+                for (int rt = 0; rt < kRT; rt++)
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ct++) acc[rt][ct] += red[rt * NCT + ct][lane];
+            __syncthreads();
         }
-        __syncthreads();
+        if (wave == 0) {
+#pragma unroll
+            for (int rt = 0; rt < kRT; rt++)
+#pragma unroll
+                for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int j = j0 + rt * 16 + M::drow(lk, r);
+                        const int col = col0 + ct * 16 + li;
+                        if (j < np && col < ncol) part[(row0 + j) * ncol + col] = acc[rt][ct][r];
+                    }
+        }
     }
 }
 
@@ -146,57 +187,66 @@ __global__ void nlb_reduce(const T *__restrict__ part, int ncol, int n_alpha,
     alpha_s[(long)r * ncol + col] = (T)(scale * gamma[r]) * s;
 }
 
+// out_b[q][col] += sum_j Phi_b[q][j] beta_b[j][col]  on 16x16x4 MFMA, computed
+// as the transposed tile (cols x points) = beta_b' Phi_b' so that the B
+// operand (Phi_b column-major, points contiguous) is a coalesced load and
+// each output register row is 16 consecutive points of one column.
 template <typename T>
 __global__ __launch_bounds__(256) void nlb_apply(T *__restrict__ out, long ldo, int ncol, int nx,
                                                  int ny, int nz, const int *__restrict__ brick_org,
                                                  const int *__restrict__ brick_np,
                                                  const long *__restrict__ brick_row,
                                                  const int *__restrict__ row_alpha,
-                                                 const T *__restrict__ phi,
-                                                 const T *__restrict__ alpha_s, int max_np)
+                                                 const T *__restrict__ phic,
+                                                 const T *__restrict__ alpha_s)
 {
-    constexpr int NC = NlbCfg<T>::NCA;
+    using M = Mfma<T>;
+    constexpr int NC = M::NCA, NCT = NC / 16, LDB = NC * 3 / 2;
     extern __shared__ unsigned char smem_raw[];
-    T *beta = reinterpret_cast<T *>(smem_raw);  // np x NC
-    const int b = blockIdx.y;
-    const int col0 = blockIdx.x * NC;
+    T *beta = reinterpret_cast<T *>(smem_raw);  // np x LDB
+    // column blocks of one brick are consecutive logical ids -> one XCD/L2
+    const int lid = xcd_remap(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int b = lid / gridDim.x;
+    const int col0 = (lid % gridDim.x) * NC;
     const int x0 = brick_org[3 * b], y0 = brick_org[3 * b + 1], z0 = brick_org[3 * b + 2];
     const int np = brick_np[b];
     const long row0 = brick_row[b];
     for (int e = threadIdx.x; e < np * NC; e += 256) {
         const int j = e / NC, c = e - j * NC;
         const int r = row_alpha[row0 + j];
-        beta[e] = (r >= 0 && col0 + c < ncol) ? alpha_s[(long)r * ncol + col0 + c] : T(0);
+        beta[j * LDB + c] = (r >= 0 && col0 + c < ncol) ? alpha_s[(long)r * ncol + col0 + c] : T(0);
     }
     __syncthreads();
-    const T *ph = phi + row0 * kBP;
-    using V4 = typename vec4<T>::type;
-    for (int q = threadIdx.x; q < kBP; q += 256) {
-        const long pt = brick_point(q, x0, y0, z0, nx, ny, nz);
-        if (pt < 0) continue;
-        T acc[NC];
+    const T *pc = phic + row0 * kBP;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    for (int pt_tile = wave; pt_tile < kBP / 16; pt_tile += kWaves) {
+        const int q0 = pt_tile * 16;
+        typename M::acc acc[NCT];
 #pragma unroll
-        for (int c = 0; c < NC; c++) acc[c] = T(0);
-        const T *prow = ph + (long)q * np;
+        for (int ct = 0; ct < NCT; ct++) acc[ct] = typename M::acc{0, 0, 0, 0};
         for (int j = 0; j < np; j += 4) {
-            const V4 p4 = *reinterpret_cast<const V4 *>(prow + j);
-            const T pv[4] = {p4.x, p4.y, p4.z, p4.w};
+            const T bq = pc[(long)(j + lk) * kBP + q0 + li];  // B (j, point)
 #pragma unroll
-            for (int a = 0; a < 4; a++) {
-                const T *bj = beta + (j + a) * NC;
+            for (int ct = 0; ct < NCT; ct++)                   // A (col, j)
+                acc[ct] = M::run(beta[(j + lk) * LDB + ct * 16 + li], bq, acc[ct]);
+        }
+        const long pt = brick_point(q0 + li, x0, y0, z0, nx, ny, nz);
+        if (pt < 0) continue;
 #pragma unroll
-                for (int c = 0; c < NC; c++) acc[c] += pv[a] * bj[c];
+        for (int ct = 0; ct < NCT; ct++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int col = col0 + ct * 16 + M::drow(lk, r);
+                if (col < ncol) out[(long)col * ldo + pt] += acc[ct][r];
             }
         }
-#pragma unroll
-        for (int c = 0; c < NC; c++)
-            if (col0 + c < ncol) out[(long)(col0 + c) * ldo + pt] += acc[c];
     }
 }
 
 template <typename T>
 int launch_vnl_typed(sgpu_context *ctx, int ncol, const T *x, long ldx, T *out, long ldo,
-                     double scale, const T *phi)
+                     double scale, const T *phi, const T *phic)
 {
     const sgpu_nonlocal &nl = ctx->nl;
     if (nl.n_bricks == 0 || nl.n_alpha == 0 || ncol <= 0) return SGPU_OK;
@@ -210,8 +260,7 @@ int launch_vnl_typed(sgpu_context *ctx, int ncol, const T *x, long ldx, T *out, 
     SGPU_CHECK(sgpu_scratch(ctx, (part_elems + alpha_elems) * sizeof(T) + 256, &buf));
     T *part = (T *)buf;
     T *alpha_s = part + part_elems;
-    constexpr int NCP = kCG * NlbCfg<T>::CT;
-    constexpr int NCA = NlbCfg<T>::NCA;
+    constexpr int NCP = Mfma<T>::NCP, NC = Mfma<T>::NCA;
     sgpu_tmark t;
     SGPU_CHECK(sgpu_tbegin(ctx, &t));
     nlb_project<T><<<dim3((ncol + NCP - 1) / NCP, nl.n_bricks), 256, 0, ctx->stream>>>(
@@ -223,10 +272,10 @@ int launch_vnl_typed(sgpu_context *ctx, int ncol, const T *x, long ldx, T *out, 
         part, ncol, nl.n_alpha, nl.arow_ptr, nl.arow_src, nl.gamma, scale * nl.dV, alpha_s);
     SGPU_HIP(hipGetLastError());
     SGPU_CHECK(sgpu_tbegin(ctx, &t));
-    const size_t shmem = (size_t)nl.max_np * NCA * sizeof(T);
-    nlb_apply<T><<<dim3((ncol + NCA - 1) / NCA, nl.n_bricks), 256, shmem, ctx->stream>>>(
+    const size_t shmem = (size_t)nl.max_np * (NC * 3 / 2) * sizeof(T);
+    nlb_apply<T><<<dim3((ncol + NC - 1) / NC, nl.n_bricks), 256, shmem, ctx->stream>>>(
         out, ldo, ncol, ctx->nx, ctx->ny, ctx->nz, nl.brick_org, nl.brick_np, nl.brick_row,
-        nl.row_alpha, phi, alpha_s, nl.max_np);
+        nl.row_alpha, phic, alpha_s);
     SGPU_HIP(hipGetLastError());
     SGPU_CHECK(sgpu_tend(ctx, &t, "nl_apply"));
     return SGPU_OK;
@@ -244,8 +293,8 @@ int upload(T **dst, const T *src, size_t n, hipStream_t s)
 
 void sgpu_free_nonlocal(sgpu_nonlocal &nl)
 {
-    void *ptrs[] = {nl.brick_org, nl.brick_np, nl.brick_row, nl.row_alpha, nl.phi64,
-                    nl.phi32,     nl.arow_ptr, nl.arow_src,  nl.gamma};
+    void *ptrs[] = {nl.brick_org, nl.brick_np, nl.brick_row, nl.row_alpha, nl.phi64, nl.phi32,
+                    nl.phic64,    nl.phic32,   nl.arow_ptr,  nl.arow_src, nl.gamma};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     nl = sgpu_nonlocal();
@@ -328,7 +377,7 @@ extern "C" int sgpu_set_nonlocal(sgpu_context *ctx, int n_influence, const int *
         s = e;
     }
     const int nb = (int)bid.size();
-    if ((size_t)max_np * 32 * sizeof(double) > 60 * 1024) {
+    if ((size_t)max_np * 24 * sizeof(double) > 60 * 1024) {
         sgpu_set_error("sgpu_set_nonlocal: %d projector columns in one brick exceed LDS staging", max_np);
         return SGPU_ERR_ARG;
     }
@@ -381,6 +430,21 @@ extern "C" int sgpu_set_nonlocal(sgpu_context *ctx, int n_influence, const int *
     SGPU_CHECK(upload(&nl.phi64, phi.data(), phi.size(), s));
     SGPU_HIP(hipMalloc((void **)&nl.phi32, std::max<size_t>(phi.size(), 1) * sizeof(float)));
     SGPU_CHECK(sgpu_convert(ctx, nl.phi64, SGPU_F64, nl.phi32, SGPU_F32, (long)phi.size()));
+    {
+        // column-major copy per brick (points contiguous) for the apply
+        std::vector<double> phic(phi.size());
+        for (int k = 0; k < nb; k++) {
+            const double *src = phi.data() + brow[k] * kBP;
+            double *dst = phic.data() + brow[k] * kBP;
+            const int npk = bnp[k];
+            for (int q = 0; q < kBP; q++)
+                for (int j = 0; j < npk; j++) dst[(long)j * kBP + q] = src[(long)q * npk + j];
+        }
+        SGPU_CHECK(upload(&nl.phic64, phic.data(), phic.size(), s));
+        SGPU_HIP(hipStreamSynchronize(s));
+    }
+    SGPU_HIP(hipMalloc((void **)&nl.phic32, std::max<size_t>(phi.size(), 1) * sizeof(float)));
+    SGPU_CHECK(sgpu_convert(ctx, nl.phic64, SGPU_F64, nl.phic32, SGPU_F32, (long)phi.size()));
     SGPU_CHECK(upload(&nl.arow_ptr, arow_ptr.data(), arow_ptr.size(), s));
     SGPU_CHECK(upload(&nl.arow_src, arow_src.data(), arow_src.size(), s));
     SGPU_CHECK(upload(&nl.gamma, gamma, (size_t)n_alpha, s));
@@ -393,7 +457,7 @@ int sgpu_launch_vnl(sgpu_context *ctx, int dtype, int ncol, const void *x, long 
 {
     if (dtype == SGPU_F64)
         return launch_vnl_typed<double>(ctx, ncol, (const double *)x, ldx, (double *)out, ldo,
-                                        scale, ctx->nl.phi64);
+                                        scale, ctx->nl.phi64, ctx->nl.phic64);
     return launch_vnl_typed<float>(ctx, ncol, (const float *)x, ldx, (float *)out, ldo, scale,
-                                   ctx->nl.phi32);
+                                   ctx->nl.phi32, ctx->nl.phic32);
 }

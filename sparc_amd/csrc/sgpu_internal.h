@@ -69,8 +69,10 @@ struct sgpu_nonlocal {
     int *brick_np = nullptr;    // padded rows per brick
     long *brick_row = nullptr;  // first brick row of each brick
     int *row_alpha = nullptr;   // alpha row of each brick row (-1: padding)
-    double *phi64 = nullptr;    // Phi_b at brick_row[b] * kBP
+    double *phi64 = nullptr;    // Phi_b row-major (kBP x np_b) at brick_row[b] * kBP
     float *phi32 = nullptr;
+    double *phic64 = nullptr;   // the same blocks column-major (np_b x kBP)
+    float *phic32 = nullptr;
     int *arow_ptr = nullptr;    // alpha row r sums brick rows arow_src[arow_ptr[r]..]
     int *arow_src = nullptr;
     double *gamma = nullptr;    // Gamma per alpha row
@@ -91,6 +93,11 @@ struct sgpu_context {
     int bcx = 0, bcy = 0, bcz = 0;
     int fdn = 0;
     double d2x[SGPU_MAX_FDN + 1] = {0}, d2y[SGPU_MAX_FDN + 1] = {0}, d2z[SGPU_MAX_FDN + 1] = {0};
+    // radius-6 tile shape (TX x TY, RY rows per thread): 0 = 32x16x2 (default,
+    // fastest measured on MI355X), 1 = 32x32x4, 2 = 64x16x4, 3 = 32x32x8,
+    // 4 = 64x32x8; set from the SPARC_AMD_STENCIL_TILE environment variable
+    // at creation
+    int stencil_cfg = 0;
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;
 
@@ -118,6 +125,19 @@ struct sgpu_context {
     // live blocks; sgpu_destroy releases their storage and orphans them
     std::vector<sgpu_block *> blocks;
 };
+
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X (linear
+// workgroup id modulo 8), and each XCD has its own L2.  xcd_remap turns the
+// linear id into a logical id such that consecutive logical ids run on the
+// same XCD, so blocks that share data (the column blocks of one brick, the
+// tiles of one column) share an L2.  Bijective for any grid size.
+constexpr int kNumXcd = 8;
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks)
+{
+    const int per = nblocks / kNumXcd, rem = nblocks % kNumXcd;
+    const int xcd = bid % kNumXcd, slot = bid / kNumXcd;
+    return (xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + slot;
+}
 
 // ---- helpers implemented in sgpu_core.hip ----
 int sgpu_scratch(sgpu_context *ctx, size_t bytes, void **p);

@@ -88,7 +88,9 @@ __global__ __launch_bounds__(256) void stencil_direct(StencilParams<T> p)
 }
 
 constexpr int kR = 6;
-
+// Radius-6 stencil: a TX x TY tile of the x-y plane plus its 6-deep halo is
+// staged in LDS per z-plane while the block marches along z; each thread owns
+// RY points of the tile and keeps their 13 z-neighbours in a register queue.
 template <typename T, int TX, int TY, int RY>
 __global__ __launch_bounds__(TX *(TY / RY)) void stencil_tiled(StencilParams<T> p, int zchunk)
 {
@@ -98,13 +100,18 @@ __global__ __launch_bounds__(TX *(TY / RY)) void stencil_tiled(StencilParams<T> 
     constexpr int LY = TY + 2 * kR;
     constexpr int NHALO = 2 * kR * TY + 2 * kR * TX;
     constexpr int HPT = (NHALO + NT - 1) / NT;
-    __shared__ T tile[LY][LX];
+    __shared__ T tile[2][LY][LX];
 
     const int tilesx = (p.nx + TX - 1) / TX;
-    const int bx = blockIdx.x % tilesx;
-    const int by = blockIdx.x / tilesx;
-    const int col = blockIdx.y;
-    const int zs = blockIdx.z * zchunk;
+    // tiles of one column (and z-chunk) are consecutive logical ids -> one
+    // XCD, so the halo rows neighbouring tiles re-read stay in that L2
+    const int lid = xcd_remap(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z),
+                              gridDim.x * gridDim.y * gridDim.z);
+    const int tid_xy = lid % gridDim.x, rest = lid / gridDim.x;
+    const int bx = tid_xy % tilesx;
+    const int by = tid_xy / tilesx;
+    const int col = rest % gridDim.y;
+    const int zs = (rest / gridDim.y) * zchunk;
     const int ze = min(p.nz, zs + zchunk);
     const int tx = threadIdx.x % TX;
     const int tyb = threadIdx.x / TX;
@@ -112,20 +119,18 @@ __global__ __launch_bounds__(TX *(TY / RY)) void stencil_tiled(StencilParams<T> 
     const long nxny = (long)p.nx * p.ny;
     const T *x = p.x + col * p.ldx;
 
-    // in-plane offsets of my interior points (-1: Dirichlet zero)
-    long coff[RY];
-    long ooff[RY];  // output offset, -1 if outside the grid
+    // in-plane offsets of my interior points (-1: Dirichlet zero / outside)
+    int coff[RY], ooff[RY];
 #pragma unroll
     for (int r = 0; r < RY; r++) {
         const int ty = tyb + r * NTY;
         const int ig = i0 + tx, jg = j0 + ty;
         const int ia = axis_index(ig, p.nx, p.bcx), ja = axis_index(jg, p.ny, p.bcy);
-        coff[r] = (ia >= 0 && ja >= 0) ? (long)ja * p.nx + ia : -1;
-        ooff[r] = (ig < p.nx && jg < p.ny) ? (long)jg * p.nx + ig : -1;
+        coff[r] = (ia >= 0 && ja >= 0) ? ja * p.nx + ia : -1;
+        ooff[r] = (ig < p.nx && jg < p.ny) ? jg * p.nx + ig : -1;
     }
     // my share of the halo: LDS slot and in-plane offset
-    int hslot[HPT];
-    long hoff[HPT];
+    int hslot[HPT], hoff[HPT];
 #pragma unroll
     for (int h = 0; h < HPT; h++) {
         const int e = threadIdx.x + h * NT;
@@ -146,39 +151,66 @@ __global__ __launch_bounds__(TX *(TY / RY)) void stencil_tiled(StencilParams<T> 
             hslot[h] = ly * LX + lx;
             const int ia = axis_index(i0 - kR + lx, p.nx, p.bcx);
             const int ja = axis_index(j0 - kR + ly, p.ny, p.bcy);
-            hoff[h] = (ia >= 0 && ja >= 0) ? (long)ja * p.nx + ia : -1;
+            hoff[h] = (ia >= 0 && ja >= 0) ? ja * p.nx + ia : -1;
         }
     }
 
+    // q[r][s] holds plane z-kR+s of point r; nq / nh are the prefetched values
+    // of the next step (plane z+1+kR of my points, halo of plane z+1), issued
+    // before this step's barrier so their latency overlaps the compute.
+    // (Fetching a plane's halo together with its interior, kR steps early,
+    // cut L2 misses by a quarter on MI355X but cost more in registers than it
+    // saved; the halo is fetched for the current plane.)
     T q[RY][2 * kR + 1];
-    // warm-up: planes zs-R .. zs+R-1 into q[.][1..2R]
 #pragma unroll
     for (int s = 0; s < 2 * kR; s++) {
         const int za = axis_index(zs - kR + s, p.nz, p.bcz);
+        const T *xa = x + (long)(za < 0 ? 0 : za) * nxny;
 #pragma unroll
-        for (int r = 0; r < RY; r++)
-            q[r][s + 1] = (za >= 0 && coff[r] >= 0) ? x[za * nxny + coff[r]] : T(0);
+        for (int r = 0; r < RY; r++) q[r][s + 1] = (za >= 0 && coff[r] >= 0) ? xa[coff[r]] : T(0);
     }
-
+    T nq[RY], nh[HPT];
+    {
+        const int zn = axis_index(zs + kR, p.nz, p.bcz);
+        const T *xn = x + (long)(zn < 0 ? 0 : zn) * nxny;
+#pragma unroll
+        for (int r = 0; r < RY; r++) nq[r] = (zn >= 0 && coff[r] >= 0) ? xn[coff[r]] : T(0);
+        const T *xz = x + (long)zs * nxny;
+#pragma unroll
+        for (int h = 0; h < HPT; h++) nh[h] = hoff[h] >= 0 ? xz[hoff[h]] : T(0);
+    }
+    int buf = 0;
     for (int z = zs; z < ze; z++) {
-        const int zn = axis_index(z + kR, p.nz, p.bcz);
 #pragma unroll
         for (int r = 0; r < RY; r++) {
 #pragma unroll
             for (int s = 0; s < 2 * kR; s++) q[r][s] = q[r][s + 1];
-            q[r][2 * kR] = (zn >= 0 && coff[r] >= 0) ? x[zn * nxny + coff[r]] : T(0);
+            q[r][2 * kR] = nq[r];
         }
-        const long zb = (long)z * nxny;
         T hv[HPT];
 #pragma unroll
-        for (int h = 0; h < HPT; h++) hv[h] = (hoff[h] >= 0) ? x[zb + hoff[h]] : T(0);
-        __syncthreads();  // previous plane fully consumed
+        for (int h = 0; h < HPT; h++) hv[h] = nh[h];
+        if (z + 1 < ze) {
+            const int zn = axis_index(z + 1 + kR, p.nz, p.bcz);
+            const T *xn = x + (long)(zn < 0 ? 0 : zn) * nxny;
 #pragma unroll
-        for (int r = 0; r < RY; r++) tile[kR + tyb + r * NTY][kR + tx] = q[r][kR];
+            for (int r = 0; r < RY; r++) nq[r] = (zn >= 0 && coff[r] >= 0) ? xn[coff[r]] : T(0);
+            const T *xz = x + (long)(z + 1) * nxny;
+#pragma unroll
+            for (int h = 0; h < HPT; h++) nh[h] = hoff[h] >= 0 ? xz[hoff[h]] : T(0);
+        }
+        // double-buffered plane: one barrier per step (a thread can only
+        // overwrite this buffer two steps later, after everyone passed the
+        // next step's barrier)
+        T(*tl)[LX] = tile[buf];
+#pragma unroll
+        for (int r = 0; r < RY; r++) tl[kR + tyb + r * NTY][kR + tx] = q[r][kR];
 #pragma unroll
         for (int h = 0; h < HPT; h++)
-            if (hslot[h] >= 0) (&tile[0][0])[hslot[h]] = hv[h];
+            if (hslot[h] >= 0) (&tl[0][0])[hslot[h]] = hv[h];
         __syncthreads();
+        buf ^= 1;
+        const long zb = (long)z * nxny;
 #pragma unroll
         for (int r = 0; r < RY; r++) {
             if (ooff[r] < 0) continue;
@@ -187,9 +219,9 @@ __global__ __launch_bounds__(TX *(TY / RY)) void stencil_tiled(StencilParams<T> 
             T res = p.w0 * xc;
 #pragma unroll
             for (int s = 1; s <= kR; s++) {
-                T rx = (tile[ly][lx - s] + tile[ly][lx + s]) * p.wx[s];
-                T ry = (tile[ly - s][lx] + tile[ly + s][lx]) * p.wy[s];
-                T rz = (q[r][kR - s] + q[r][kR + s]) * p.wz[s];
+                const T rx = (tl[ly][lx - s] + tl[ly][lx + s]) * p.wx[s];
+                const T ry = (tl[ly - s][lx] + tl[ly + s][lx]) * p.wy[s];
+                const T rz = (q[r][kR - s] + q[r][kR + s]) * p.wz[s];
                 res += rx + ry + rz;
             }
             const long o = zb + ooff[r];
@@ -199,6 +231,19 @@ __global__ __launch_bounds__(TX *(TY / RY)) void stencil_tiled(StencilParams<T> 
             p.out[col * p.ldo + o] = val;
         }
     }
+}
+
+template <typename T, int TX, int TY, int RY>
+void launch_tiled(sgpu_context *ctx, const StencilParams<T> &p, int ncol)
+{
+    const int tiles = ((ctx->nx + TX - 1) / TX) * ((ctx->ny + TY - 1) / TY);
+    // enough blocks to fill the chip: split z when there are few columns
+    long want = (2048 + (long)tiles * ncol - 1) / ((long)tiles * ncol);
+    int nzc = (int)std::max(1L, std::min<long>(want, std::max(1, ctx->nz / 16)));
+    int zchunk = (ctx->nz + nzc - 1) / nzc;
+    nzc = (ctx->nz + zchunk - 1) / zchunk;
+    dim3 grid(tiles, ncol, nzc);
+    stencil_tiled<T, TX, TY, RY><<<grid, TX * (TY / RY), 0, ctx->stream>>>(p, zchunk);
 }
 
 template <typename T>
@@ -230,14 +275,13 @@ int launch_typed(sgpu_context *ctx, int ncol, double a, double bdiag, double c, 
         return SGPU_ERR_ARG;
     }
     if (allow_tiled && ctx->fdn == kR) {
-        constexpr int TX = 32, TY = 16, RY = 2;
-        const int tiles = ((ctx->nx + TX - 1) / TX) * ((ctx->ny + TY - 1) / TY);
-        long want = (2048 + (long)tiles * ncol - 1) / ((long)tiles * ncol);
-        int nzc = (int)std::max(1L, std::min<long>(want, std::max(1, ctx->nz / 16)));
-        int zchunk = (ctx->nz + nzc - 1) / nzc;
-        nzc = (ctx->nz + zchunk - 1) / zchunk;
-        dim3 grid(tiles, ncol, nzc);
-        stencil_tiled<T, TX, TY, RY><<<grid, TX * (TY / RY), 0, ctx->stream>>>(p, zchunk);
+        switch (ctx->stencil_cfg) {
+            case 1: launch_tiled<T, 32, 32, 4>(ctx, p, ncol); break;
+            case 2: launch_tiled<T, 64, 16, 4>(ctx, p, ncol); break;
+            case 3: launch_tiled<T, 32, 32, 8>(ctx, p, ncol); break;
+            case 4: launch_tiled<T, 64, 32, 8>(ctx, p, ncol); break;
+            default: launch_tiled<T, 32, 16, 2>(ctx, p, ncol); break;
+        }
     } else {
         dim3 grid((unsigned)((ctx->nd + 255) / 256), ncol);
         switch (ctx->fdn) {

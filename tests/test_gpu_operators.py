@@ -35,6 +35,20 @@ def test_lap_plus_diag_fp64(gpu, oracle_mod, n, bc):
             assert rel_err(y, yo) < 1e-13, (a, b, c)
 
 
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("n,bc", [((40, 35, 13), (0, 0, 0)), ((70, 33, 9), (0, 1, 1)),
+                                  ((7, 6, 30), (0, 0, 0))])
+def test_stencil_tile_shapes(gpu, oracle_mod, monkeypatch, cfg, n, bc):
+    """Every radius-6 tile shape against the oracle (partial tiles, wrap, walls)."""
+    monkeypatch.setenv("SPARC_AMD_STENCIL_TILE", str(cfg))
+    g, veff, nl, rng = small_case(n=n, bc=bc, seed=30 + cfg)
+    o = oracle_mod.Oracle(g, veff)
+    x = rng.standard_normal((g.nd, 2))
+    with gpu.Context(g) as ctx:
+        y = ctx.Lap_plus_diag_vec_mult(x, -0.5, 1.0, 0.4, veff)
+    assert rel_err(y, o.lap_plus_diag(x, -0.5, 1.0, 0.4, veff)) < 1e-13
+
+
 @pytest.mark.parametrize("order", [2, 4, 8, 16])
 def test_lap_other_orders(gpu, oracle_mod, order):
     g, veff, nl, rng = small_case(n=(12, 11, 10), order=order, seed=2)
