@@ -1,0 +1,161 @@
+"""Pin the FP64 oracle's operators: FD weights, stencil, Vnl, Hamiltonian.
+
+Each check is independent of the oracle's own formula: moment conditions of
+the FD weights, the exact Fourier symbol of the periodic stencil, brute-force
+dense matrices on tiny grids, and the reference's own compiled stencil
+(oracle/_ref, built from SPARC's lapVecRoutines.c) where it is available.
+"""
+import itertools
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sparc_amd.workloads import Grid
+from tests.helpers import rel_err, small_case, split_images
+
+
+@pytest.mark.parametrize("fdn", [1, 2, 3, 4, 5, 6, 7, 8])
+def test_fd_weights_moment_conditions(fdn):
+    """A central 2nd-derivative stencil of radius n is exact on x^k, k <= 2n+1:
+    sum_p w_p p^k = 2 for k = 2 and 0 for the other even k (odd k vanish by
+    symmetry).  A wrong sign, index or factorial fails this."""
+    w = O.fd_weights(fdn)
+    for k in range(0, 2 * fdn + 1, 2):
+        s = (w[0] if k == 0 else 0.0) + sum(2 * w[p] * p ** k for p in range(1, fdn + 1))
+        assert abs(s - (2.0 if k == 2 else 0.0)) < 1e-9 * max(1, fdn ** k), (k, s)
+
+
+def test_fd_weights_order12_exact_rationals():
+    """Order-12 weights equal the unique solution of the moment system in exact
+    rational arithmetic (solved here independently with Fractions)."""
+    n = 6
+    # unknowns w_1..w_6: sum_p 2 w_p p^(2j) = delta_{j,1}*2 for j = 1..6
+    A = [[Fraction(2 * p ** (2 * j)) for p in range(1, n + 1)] for j in range(1, n + 1)]
+    b = [Fraction(2 if j == 1 else 0) for j in range(1, n + 1)]
+    # Gauss-Jordan in fractions
+    M = [row + [bb] for row, bb in zip(A, b)]
+    for c in range(n):
+        piv = next(r for r in range(c, n) if M[r][c] != 0)
+        M[c], M[piv] = M[piv], M[c]
+        M[c] = [v / M[c][c] for v in M[c]]
+        for r in range(n):
+            if r != c and M[r][c] != 0:
+                M[r] = [a - M[r][c] * bcol for a, bcol in zip(M[r], M[c])]
+    exact = [M[r][n] for r in range(n)]
+    w0 = -2 * sum(exact)
+    w = O.fd_weights(6)
+    assert abs(w[0] - float(w0)) < 1e-14
+    for p in range(1, 7):
+        assert abs(w[p] - float(exact[p - 1])) < 1e-14
+
+
+def test_mesh2chebdegree_matches_published_fit_points():
+    # the fit passes through (h, npl) = (0.1,50), (0.2,35), (0.4,20), (0.7,14)
+    for h, n in ((0.1, 50), (0.2, 35), (0.4, 20), (0.7, 14), (1.0, 14)):
+        assert O.mesh2chebdegree(h) == n
+    assert O.mesh2chebdegree(0.3946) == 20  # the 26^3 / 78^3 silicon meshes
+
+
+@pytest.mark.parametrize("n", [(12, 10, 9), (7, 8, 6)])
+def test_stencil_fourier_symbol(n):
+    """Periodic FD Laplacian on plane waves: exact eigenvalue = symbol(k)."""
+    g = Grid(*n, 0.41, 0.37, 0.52)
+    o = O.Oracle(g)
+    X, Y, Z = g.coords()
+    rng = np.random.default_rng(0)
+    sym = O.stencil_symbol(g, o.coeffs)
+    for _ in range(6):
+        kx, ky, kz = (rng.integers(0, m) for m in n)
+        k = 2 * np.pi * np.array([kx / (n[0] * g.hx), ky / (n[1] * g.hy), kz / (n[2] * g.hz)])
+        v = np.cos(k[0] * X + k[1] * Y + k[2] * Z + 0.3)
+        lv = o.lap_vec_mult(v)[:, 0]
+        lam = sym[kz, ky, kx]
+        assert np.max(np.abs(lv - lam * v)) < 1e-11 * max(1.0, abs(lam))
+
+
+def test_max_eig_mhalf_lap_closed_form():
+    g = Grid(14, 11, 9, 0.4, 0.45, 0.5)
+    o = O.Oracle(g)
+    sym = O.stencil_symbol(g, o.coeffs)
+    # the closed form uses the Nyquist-most wavenumber of each axis
+    assert abs(o.max_eig_mhalf_lap() - (-0.5 * sym.min())) < 1e-10
+
+
+def _dense_lap(g, coeffs):
+    """Brute-force dense Laplacian with explicit wrap / zero walls."""
+    n = (g.nx, g.ny, g.nz)
+    bc = (g.bcx, g.bcy, g.bcz)
+    nd = g.nd
+    L = np.zeros((nd, nd))
+    idx = lambda i, j, k: (k * g.ny + j) * g.nx + i
+    for i, j, k in itertools.product(range(g.nx), range(g.ny), range(g.nz)):
+        row = idx(i, j, k)
+        L[row, row] += coeffs[0][0] + coeffs[1][0] + coeffs[2][0]
+        for ax in range(3):
+            for p in range(1, len(coeffs[ax])):
+                for sgn in (-1, 1):
+                    c = [i, j, k]
+                    c[ax] += sgn * p
+                    if 0 <= c[ax] < n[ax]:
+                        L[row, idx(*c)] += coeffs[ax][p]
+                    elif not bc[ax]:
+                        c[ax] %= n[ax]
+                        L[row, idx(*c)] += coeffs[ax][p]
+    return L
+
+
+@pytest.mark.parametrize("bc", [(0, 0, 0), (1, 1, 1), (0, 1, 0)])
+def test_stencil_bruteforce_dense(bc):
+    g = Grid(7, 6, 8, 0.5, 0.45, 0.4, *bc)
+    o = O.Oracle(g)
+    L = _dense_lap(g, o.coeffs)
+    rng = np.random.default_rng(1)
+    v = rng.standard_normal(g.nd)
+    x = rng.standard_normal((g.nd, 3))
+    y = o.lap_plus_diag(x, -0.5, 1.0, 0.7, v)
+    ref = -0.5 * L @ x + (v[:, None] + 0.7) * x
+    assert rel_err(y, ref) < 1e-13
+
+
+@pytest.mark.skipif(O.ref_lib() is None, reason="reference stencil (oracle/_ref) not built here")
+@pytest.mark.parametrize("n,bc", [((13, 11, 9), (0, 1, 0)), ((16, 16, 16), (0, 0, 0)),
+                                  ((9, 12, 10), (1, 1, 1)), ((6, 6, 6), (0, 0, 0))])
+def test_oracle_matches_reference_stencil(n, bc):
+    """The oracle's Lap_plus_diag equals SPARC's own compiled routine bit-for-bit."""
+    g = Grid(*n, 0.4, 0.45, 0.5, *bc)
+    o = O.Oracle(g)
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal((g.nd, 3))
+    v = rng.standard_normal(g.nd)
+    for a, b, c in ((1.0, 0.0, 0.0), (-0.5, 1.0, 0.3), (2.0, 0.5, -1.0)):
+        y1 = o.lap_plus_diag(x, a, b, c, v)
+        y2 = O.ref_lap_plus_diag(g, o.coeffs, x, a, b, c, v)
+        assert np.array_equal(y1, y2), (a, b, c)
+
+
+def test_vnl_matches_dense_and_is_symmetric():
+    g, veff, nl, rng = small_case(n=(9, 8, 7), n_sites=3, rc=1.7, seed=3, lmax=2)
+    o = O.Oracle(g, veff, nl)
+    V = nl.dense(g.nd)
+    assert np.allclose(V, V.T, atol=1e-14)
+    x = rng.standard_normal((g.nd, 5))
+    assert rel_err(o.vnl_vec_mult(x), V @ x) < 1e-13
+
+
+def test_vnl_periodic_images_share_rows():
+    g, veff, nl, rng = small_case(n=(12, 11, 10), n_sites=4, rc=2.3, seed=4)
+    o1 = O.Oracle(g, veff, nl)
+    o2 = O.Oracle(g, veff, split_images(nl, 2))
+    x = rng.standard_normal((g.nd, 4))
+    assert rel_err(o2.vnl_vec_mult(x), o1.vnl_vec_mult(x)) < 1e-14
+
+
+def test_hamiltonian_is_symmetric_operator():
+    g, veff, nl, rng = small_case(n=(8, 7, 6), n_sites=2, rc=1.6, seed=5)
+    o = O.Oracle(g, veff, nl)
+    H = o.hamiltonian(np.eye(g.nd))
+    assert np.max(np.abs(H - H.T)) < 1e-10 * np.abs(H).max()
+    dense = -0.5 * _dense_lap(g, o.coeffs) + np.diag(veff) + nl.dense(g.nd)
+    assert rel_err(H, dense) < 1e-13
