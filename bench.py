@@ -42,6 +42,17 @@ def stencil_bytes(nd, ns, m, s):
     return s * nd * ns * (2 + 3 * (m - 1)) + m * s * nd
 
 
+def profiled_traffic(kernel="stencil_tiled"):
+    """HBM bytes per launch of the dominant kernel, from the committed
+    FETCH_SIZE / WRITE_SIZE summary (profiles/), or None."""
+    import re
+    p = ROOT / "profiles" / "filter_operator_96x512_deg24_hbm_traffic.md"
+    if not p.exists():
+        return None
+    m = re.search(rf"traffic_bytes_per_launch {kernel}:\s*([0-9.eE+]+)", p.read_text())
+    return float(m.group(1)) if m else None
+
+
 def bounds_lanczos(ctx, wl):
     from sparc_amd.workloads import uniform_block
     x0 = uniform_block(wl.grid.nd, 1, wl.lanczos_seed, 0.0, 1.0)[:, 0]
@@ -178,7 +189,8 @@ def main(argv=None):
         "filter_ms": {"f32": round(ms32, 3)},
         "filter_gbs": {"f32": round(gbs32, 2)},
         "roofline": {"bound": "hbm", "achieved": round(st_gbs, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(st_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(st_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic() if (args.n, ns, m) == (96, 512, 24) else None,
                      "kernel": "stencil_tiled<float,32,16,2>",
                      "per_launch_algorithmic_bytes": stencil_bytes(nd, ns, m, 4) / m},
         "finite": r32["finite"],
