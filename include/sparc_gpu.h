@@ -175,6 +175,59 @@ int sgpu_lanczos(sgpu_context *ctx, const double *x0, double tol_min, double tol
 int sgpu_chefsi_dev(sgpu_context *ctx, sgpu_block *X, sgpu_block *Y, int m, double a, double b,
                     double a0, double *lambda);
 
+/* ---- self-consistent field loop ------------------------------------- */
+
+/* Parameters of the density-mixing SCF (defaults of SPARC in brackets,
+ * src/initialization.c:677-989,2233-2608). */
+typedef struct {
+    int nstates;            /* Nstates */
+    int cheb_degree;        /* ChebDegree (Mesh2ChebDegree(h)) */
+    int rho_trigger;        /* CheFSI passes in the first SCF [4] */
+    int max_scf;            /* MAXIT_SCF [100] */
+    int min_scf;            /* MINIT_SCF [2] */
+    double tol_scf;         /* TOL_SCF */
+    double tol_poisson;     /* TOL_POISSON (relative to ||rhs||) */
+    int maxit_poisson;      /* MAXIT_POISSON [3000] */
+    double tol_lanczos;     /* TOL_LANCZOS [1e-2] */
+    double beta;            /* 1/(k_B T) [1/Ha] */
+    int smear_type;         /* elec_T_type: 0 Fermi-Dirac, 1 Gaussian [1] */
+    double nelec;           /* electrons (-NegCharge) */
+    double occfac;          /* 2 for spin-unpolarized */
+    int mix_history;        /* MixingHistory [7], <= 8 */
+    int pulay_freq;         /* PulayFrequency [1] */
+    double mix_beta;        /* MixingParameter [0.3] */
+    double mix_omega;       /* MixingParameterSimple [= MixingParameter] */
+    double kerker_ktf;      /* precond_kerker_kTF [1.0] */
+    double kerker_thresh;   /* precond_kerker_thresh [0.1] */
+    double tol_precond;     /* TOL_PRECOND [h^2 1e-3] */
+    double Esc;             /* self + correction energy of the pseudocharges */
+    int dtype;              /* SGPU_F64 or SGPU_F32 vector blocks (filter, Gram, rotation) */
+    unsigned long long psi_seed;      /* initial orbitals uniform [-0.5, 0.5) */
+    unsigned long long lanczos_seed;  /* Lanczos start vector uniform [0, 1) */
+} sgpu_scf_params;
+
+typedef struct {
+    int scf_iters;          /* SCF iterations done */
+    int converged;
+    double Etot, Eband, Exc, E1, E2, E3, Entropy, Efermi;  /* last iteration (src/energy.c:165) */
+    double scf_error;       /* last |rho_out - rho_in| / |rho_out| */
+    double mean_iter_ms;    /* mean wall time of iterations 2..N */
+    double total_ms;
+    long poisson_iters, kerker_iters;  /* AAR iterations summed over the run */
+} sgpu_scf_result;
+
+/* Density-mixing SCF with CheFSI: replaces scf / scf_loop
+ * (src/electronicGroundState.c:607-1147) for one spin-unpolarized Gamma-point
+ * system whose grid, Veff storage and nonlocal projectors live in ctx.  b is
+ * the pseudocharge density and rho0 the initial electron density (Nd each,
+ * host).  Outputs: lambda (nstates eigenvalues of the last pass), optional
+ * final mixed density / electrostatic potential (Nd each) and per-iteration
+ * wall time, SCF error and total energy (max_scf entries each).  Returns
+ * SGPU_NOT_CONVERGED if max_scf was reached. */
+int sgpu_scf_run(sgpu_context *ctx, const sgpu_scf_params *par, const double *b,
+                 const double *rho0, double *lambda, double *rho_out, double *phi_out,
+                 double *iter_ms, double *iter_err, double *iter_etot, sgpu_scf_result *res);
+
 /* ---- timing ---------------------------------------------------------- */
 
 /* Average device time (ms) of the last timed launches of the named kernel

@@ -146,6 +146,8 @@ void jacobi_eig(int n, double *A, double *w, double *V)
     for (int i = 0; i < n; i++) w[i] = A[i + i * n];
 }
 
+}  // namespace
+
 // Minimum-norm least-squares solution of (F'F) G = F'f, singular values
 // below eps * s_max dropped: the result LAPACKE_dgelsd(rcond = -1) returns in
 // AndersonExtrapCoeff (src/mixing.c:135).
@@ -167,39 +169,74 @@ void anderson_coeff(int m, const double *FtF, const double *Ftf, double *G)
     }
 }
 
-struct aar_ws {
-    double *r, *x_old, *f, *f_old, *Xh, *Fh, *b, *x, *partial;
-    long ld;
-};
-
-}  // namespace
-
-extern "C" int sgpu_AAR(sgpu_context *ctx, sgpu_res_fn res, sgpu_precond_fn precond, void *user,
-                        double c, int N, double *x_host, const double *b_host, double omega,
-                        double beta, int m, int p, double tol, int max_iter, int *iters,
-                        double *relres)
+// G = pinv(F'F) F'f for device F (n x m, column stride ldf) and f: one pass
+// computes F'F and F'f together, the m x m solve runs on the host in FP64.
+int sgpu_anderson_coeff_dev(sgpu_context *ctx, const double *F, long ldf, int m, const double *f,
+                            long n, double *G)
 {
-    if (!ctx || !x_host || !b_host || N != ctx->nd || m < 1 || m > kAarMaxM || p < 1 ||
-        max_iter < 0) {
-        sgpu_set_error("sgpu_AAR: bad arguments (N %d, Nd %ld, m %d, p %d)", N, ctx ? ctx->nd : 0L,
-                       m, p);
+    if (m < 1 || m > kAarMaxM) {
+        sgpu_set_error("anderson: history %d outside 1..%d", m, kAarMaxM);
         return SGPU_ERR_ARG;
     }
-    SGPU_HIP(hipSetDevice(ctx->device));
-    const long n = N;
-    const long ld = ctx->ld;
     const int gram_blocks = 256;
     const int nv = kAarMaxM * (kAarMaxM + 1) / 2 + kAarMaxM;
-    aar_ws w;
-    w.ld = ld;
-    double *mem = nullptr;
-    // r, x_old, f, f_old, b, x, X (m), F (m), partials
-    const size_t elems = (size_t)(6 + 2 * m) * ld + (size_t)gram_blocks * nv + nv;
-    SGPU_HIP(hipMalloc((void **)&mem, elems * sizeof(double)));
-    w.r = mem; w.x_old = mem + ld; w.f = mem + 2 * ld; w.f_old = mem + 3 * ld;
-    w.b = mem + 4 * ld; w.x = mem + 5 * ld; w.Xh = mem + 6 * ld; w.Fh = w.Xh + (long)m * ld;
-    w.partial = w.Fh + (long)m * ld;
-    double *gram_out = w.partial + (size_t)gram_blocks * nv;
+    double *partial = nullptr;
+    SGPU_CHECK(sgpu_reduce_buf(ctx, (size_t)gram_blocks * nv + nv, &partial));
+    double *out = partial + (size_t)gram_blocks * nv;
+    aar_gram<kAarMaxM><<<gram_blocks, 256, 0, ctx->stream>>>(F, ldf, f, m, n, partial);
+    aar_gram_final<<<1, 64, 0, ctx->stream>>>(partial, gram_blocks, nv, out);
+    SGPU_HIP(hipGetLastError());
+    SGPU_HIP(hipMemcpyAsync(ctx->host_red, out, nv * sizeof(double), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    SGPU_HIP(hipStreamSynchronize(ctx->stream));
+    double FtF[kAarMaxM * kAarMaxM], Ftf[kAarMaxM];
+    int k = 0;
+    for (int a = 0; a < kAarMaxM; a++)
+        for (int bb = a; bb < kAarMaxM; bb++, k++)
+            if (a < m && bb < m) FtF[a + bb * m] = FtF[bb + a * m] = ctx->host_red[k];
+    for (int a = 0; a < kAarMaxM; a++, k++)
+        if (a < m) Ftf[a] = ctx->host_red[k];
+    anderson_coeff(m, FtF, Ftf, G);
+    return SGPU_OK;
+}
+
+// x = xk - X G + beta (f - F G) on device vectors (history stride ldh).
+int sgpu_anderson_update_dev(sgpu_context *ctx, double *x, const double *xk, const double *f,
+                             const double *Xh, const double *Fh, long ldh, int m, const double *G,
+                             double beta, long n)
+{
+    double g[kAarMaxM] = {0};
+    for (int a = 0; a < m && a < kAarMaxM; a++) g[a] = G[a];
+    aar_anderson_update<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(
+        x, xk, f, Xh, Fh, ldh, m, g[0], g[1], g[2], g[3], g[4], g[5], g[6], g[7], beta, n);
+    SGPU_HIP(hipGetLastError());
+    return SGPU_OK;
+}
+
+// AAR on device vectors x (in/out) and b, both of length N = Nd.
+int sgpu_aar_device(sgpu_context *ctx, sgpu_res_fn res, sgpu_precond_fn precond, void *user,
+                    double c, int N, double *x, const double *b, double omega, double beta, int m,
+                    int p, double tol, int max_iter, int *iters, double *relres)
+{
+    if (!ctx || !x || !b || N != ctx->nd || m < 1 || m > kAarMaxM || p < 1 || max_iter < 0) {
+        sgpu_set_error("AAR: bad arguments (N %d, Nd %ld, m %d, p %d)", N, ctx ? ctx->nd : 0L, m, p);
+        return SGPU_ERR_ARG;
+    }
+    const long n = N;
+    const long ld = ctx->ld;
+    // r, x_old, f, f_old, X (m), F (m): cached on the context
+    const size_t elems = (size_t)(4 + 2 * m) * ld;
+    if (elems > ctx->aar_elems) {
+        SGPU_HIP(hipStreamSynchronize(ctx->stream));
+        if (ctx->aar_mem) SGPU_HIP(hipFree(ctx->aar_mem));
+        ctx->aar_mem = nullptr;
+        ctx->aar_elems = 0;
+        SGPU_HIP(hipMalloc((void **)&ctx->aar_mem, elems * sizeof(double)));
+        ctx->aar_elems = elems;
+    }
+    double *mem = ctx->aar_mem;
+    double *r = mem, *x_old = mem + ld, *f = mem + 2 * ld, *f_old = mem + 3 * ld;
+    double *Xh = mem + 4 * ld, *Fh = Xh + (long)m * ld;
     const unsigned nb = (unsigned)((n + 255) / 256);
     int it = 0;
     double rnorm = 0.0, bnorm = 0.0;
@@ -208,79 +245,86 @@ extern "C" int sgpu_AAR(sgpu_context *ctx, sgpu_res_fn res, sgpu_precond_fn prec
 
     auto residual = [&]() -> int {
         if (res) {
-            res(user, N, c, w.x, w.b, w.r);
+            res(user, N, c, x, b, r);
         } else {
             // r = b + (Lap + c) x
             sgpu_tmark t;
             SGPU_CHECK(sgpu_tbegin(ctx, &t));
-            SGPU_CHECK(sgpu_launch_stencil(ctx, SGPU_F64, 1, 1.0, 0.0, c, nullptr, w.x, ld, w.r, ld,
-                                           w.b, ld, 1.0, 1.0, true));
+            SGPU_CHECK(sgpu_launch_stencil(ctx, SGPU_F64, 1, 1.0, 0.0, c, nullptr, x, ld, r, ld, b,
+                                           ld, 1.0, 1.0, true));
             SGPU_CHECK(sgpu_tend(ctx, &t, "aar_residual"));
         }
         return SGPU_OK;
     };
-    auto body = [&]() -> int {
-        SGPU_HIP(hipMemsetAsync(w.Xh, 0, 2 * (size_t)m * ld * sizeof(double), ctx->stream));
-        SGPU_HIP(hipMemcpyAsync(w.x, x_host, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-        SGPU_HIP(hipMemcpyAsync(w.b, b_host, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-        SGPU_HIP(hipMemcpyAsync(w.x_old, w.x, n * sizeof(double), hipMemcpyDeviceToDevice,
-                                ctx->stream));
-        const double *pa[1] = {w.b}, *pb[1] = {w.b};
-        SGPU_CHECK(sgpu_dot_many(ctx, N, 1, pa, pb, &bnorm));
-        bnorm = std::sqrt(bnorm);
-        SGPU_CHECK(residual());
-        const double atol = tol * bnorm;
-        rnorm = atol + 1.0;
-        while (rnorm > atol && it < max_iter) {
-            const bool anderson = ((it + 1) % p == 0) && it > 0;
-            const int ih = it > 0 ? (it - 1) % m : 0;
-            if (precond) precond(user, N, c, w.r, w.f);
-            aar_fused<<<nb, 256, 0, ctx->stream>>>(w.r, w.f, precond ? 0.0 : jac, w.x, w.x_old,
-                                                   w.f_old, w.Xh + (long)ih * ld,
-                                                   w.Fh + (long)ih * ld, it > 0 ? 1 : 0,
-                                                   anderson ? 0.0 : omega, n);
-            SGPU_HIP(hipGetLastError());
-            if (anderson) {
-                aar_gram<kAarMaxM><<<gram_blocks, 256, 0, ctx->stream>>>(w.Fh, ld, w.f, m, n,
-                                                                         w.partial);
-                aar_gram_final<<<1, 64, 0, ctx->stream>>>(w.partial, gram_blocks, nv, gram_out);
-                SGPU_HIP(hipGetLastError());
-                SGPU_HIP(hipMemcpyAsync(ctx->host_red, gram_out, nv * sizeof(double),
-                                        hipMemcpyDeviceToHost, ctx->stream));
-                SGPU_HIP(hipStreamSynchronize(ctx->stream));
-                double FtF[kAarMaxM * kAarMaxM], Ftf[kAarMaxM], G[kAarMaxM] = {0};
-                int k = 0;
-                for (int a = 0; a < kAarMaxM; a++)
-                    for (int bb = a; bb < kAarMaxM; bb++, k++)
-                        if (a < m && bb < m) FtF[a + bb * m] = FtF[bb + a * m] = ctx->host_red[k];
-                for (int a = 0; a < kAarMaxM; a++, k++)
-                    if (a < m) Ftf[a] = ctx->host_red[k];
-                anderson_coeff(m, FtF, Ftf, G);
-                aar_anderson_update<<<nb, 256, 0, ctx->stream>>>(
-                    w.x, w.x_old, w.f, w.Xh, w.Fh, ld, m, G[0], G[1], G[2], G[3], G[4], G[5], G[6],
-                    G[7], beta, n);
-                SGPU_HIP(hipGetLastError());
-                SGPU_CHECK(residual());
-                pa[0] = w.r; pb[0] = w.r;
-                SGPU_CHECK(sgpu_dot_many(ctx, N, 1, pa, pb, &rnorm));
-                rnorm = std::sqrt(rnorm);
-            } else {
-                SGPU_CHECK(residual());
-            }
-            it++;
+    SGPU_HIP(hipMemsetAsync(Xh, 0, 2 * (size_t)m * ld * sizeof(double), ctx->stream));
+    SGPU_HIP(hipMemcpyAsync(x_old, x, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    const double *pa[1] = {b}, *pb[1] = {b};
+    SGPU_CHECK(sgpu_dot_many(ctx, N, 1, pa, pb, &bnorm));
+    bnorm = std::sqrt(bnorm);
+    SGPU_CHECK(residual());
+    const double atol = tol * bnorm;
+    rnorm = atol + 1.0;
+    while (rnorm > atol && it < max_iter) {
+        const bool anderson = ((it + 1) % p == 0) && it > 0;
+        const int ih = it > 0 ? (it - 1) % m : 0;
+        if (precond) precond(user, N, c, r, f);
+        aar_fused<<<nb, 256, 0, ctx->stream>>>(r, f, precond ? 0.0 : jac, x, x_old, f_old,
+                                               Xh + (long)ih * ld, Fh + (long)ih * ld,
+                                               it > 0 ? 1 : 0, anderson ? 0.0 : omega, n);
+        SGPU_HIP(hipGetLastError());
+        if (anderson) {
+            double G[kAarMaxM] = {0};
+            SGPU_CHECK(sgpu_anderson_coeff_dev(ctx, Fh, ld, m, f, n, G));
+            SGPU_CHECK(sgpu_anderson_update_dev(ctx, x, x_old, f, Xh, Fh, ld, m, G, beta, n));
+            SGPU_CHECK(residual());
+            pa[0] = r; pb[0] = r;
+            SGPU_CHECK(sgpu_dot_many(ctx, N, 1, pa, pb, &rnorm));
+            rnorm = std::sqrt(rnorm);
+        } else {
+            SGPU_CHECK(residual());
         }
-        SGPU_HIP(hipMemcpyAsync(x_host, w.x, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        double r2 = 0.0;
-        pa[0] = w.r; pb[0] = w.r;
-        SGPU_CHECK(sgpu_dot_many(ctx, N, 1, pa, pb, &r2));
-        if (relres) *relres = bnorm > 0 ? std::sqrt(r2) / bnorm : 0.0;
-        SGPU_HIP(hipStreamSynchronize(ctx->stream));
-        return SGPU_OK;
-    };
-    int st = body();
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipFree(mem);
+        it++;
+    }
+    double r2 = 0.0;
+    pa[0] = r; pb[0] = r;
+    SGPU_CHECK(sgpu_dot_many(ctx, N, 1, pa, pb, &r2));
+    if (relres) *relres = bnorm > 0 ? std::sqrt(r2) / bnorm : 0.0;
     if (iters) *iters = it;
-    if (st < 0) return st;
-    return (rnorm > tol * bnorm && it >= max_iter) ? SGPU_NOT_CONVERGED : SGPU_OK;
+    return (rnorm > atol && it >= max_iter) ? SGPU_NOT_CONVERGED : SGPU_OK;
+}
+
+extern "C" int sgpu_AAR(sgpu_context *ctx, sgpu_res_fn res, sgpu_precond_fn precond, void *user,
+                        double c, int N, double *x_host, const double *b_host, double omega,
+                        double beta, int m, int p, double tol, int max_iter, int *iters,
+                        double *relres)
+{
+    if (!ctx || !x_host || !b_host || N != ctx->nd) {
+        sgpu_set_error("sgpu_AAR: bad arguments (N %d, Nd %ld)", N, ctx ? ctx->nd : 0L);
+        return SGPU_ERR_ARG;
+    }
+    SGPU_HIP(hipSetDevice(ctx->device));
+    const long ld = ctx->ld;
+    double *xb = nullptr;
+    SGPU_HIP(hipMalloc((void **)&xb, 2 * ld * sizeof(double)));
+    double *xd = xb, *bd = xb + ld;
+    int st = SGPU_OK;
+    if (hipMemcpyAsync(xd, x_host, (size_t)N * sizeof(double), hipMemcpyHostToDevice, ctx->stream) !=
+            hipSuccess ||
+        hipMemcpyAsync(bd, b_host, (size_t)N * sizeof(double), hipMemcpyHostToDevice, ctx->stream) !=
+            hipSuccess) {
+        sgpu_set_error("sgpu_AAR: upload failed");
+        st = SGPU_ERR_HIP;
+    }
+    if (st == SGPU_OK)
+        st = sgpu_aar_device(ctx, res, precond, user, c, N, xd, bd, omega, beta, m, p, tol, max_iter,
+                             iters, relres);
+    if (st >= 0 && (hipMemcpyAsync(x_host, xd, (size_t)N * sizeof(double), hipMemcpyDeviceToHost,
+                                   ctx->stream) != hipSuccess ||
+                    hipStreamSynchronize(ctx->stream) != hipSuccess)) {
+        sgpu_set_error("sgpu_AAR: download failed");
+        st = SGPU_ERR_HIP;
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(xb);
+    return st;
 }

@@ -124,6 +124,7 @@ extern "C" int sgpu_destroy(sgpu_context *ctx)
     if (ctx->veff32) (void)hipFree(ctx->veff32);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->red_buf) (void)hipFree(ctx->red_buf);
+    if (ctx->aar_mem) (void)hipFree(ctx->aar_mem);
     if (ctx->host_red) (void)hipHostFree(ctx->host_red);
     for (auto &pe : ctx->pending) {
         (void)hipEventDestroy(pe.a);

@@ -113,6 +113,8 @@ struct sgpu_context {
     double *red_buf = nullptr;      // reduction partials
     size_t red_elems = 0;
     double *host_red = nullptr;     // pinned host mirror of small results
+    double *aar_mem = nullptr;      // AAR work vectors (r, x_old, f, f_old, X, F)
+    size_t aar_elems = 0;
 
     bool timing = false;
     std::map<std::string, sgpu_timer_acc> timers;
@@ -162,6 +164,17 @@ void sgpu_free_nonlocal(sgpu_nonlocal &nl);
 // out += scale * Vnl x  (alpha computed from x); dtype of x/out.
 int sgpu_launch_vnl(sgpu_context *ctx, int dtype, int ncol, const void *x, long ldx, void *out,
                     long ldo, double scale);
+// ---- aar.hip ----
+void anderson_coeff(int m, const double *FtF, const double *Ftf, double *G);
+int sgpu_anderson_coeff_dev(sgpu_context *ctx, const double *F, long ldf, int m, const double *f,
+                            long n, double *G);
+int sgpu_anderson_update_dev(sgpu_context *ctx, double *x, const double *xk, const double *f,
+                             const double *Xh, const double *Fh, long ldh, int m, const double *G,
+                             double beta, long n);
+// AAR on device vectors (x in/out, b), returns SGPU_OK / SGPU_NOT_CONVERGED / error.
+int sgpu_aar_device(sgpu_context *ctx, sgpu_res_fn res, sgpu_precond_fn precond, void *user,
+                    double c, int N, double *x, const double *b, double omega, double beta, int m,
+                    int p, double tol, int max_iter, int *iters, double *relres);
 // ---- blas.hip ----
 int sgpu_dot_many(sgpu_context *ctx, int n, int npairs, const double *const *a,
                   const double *const *b, double *host_out);

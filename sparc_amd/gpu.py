@@ -47,6 +47,27 @@ _ip = C.POINTER(C.c_int)
 _lp = C.POINTER(C.c_long)
 _vp = C.c_void_p
 
+class SCFParams(C.Structure):
+    _fields_ = [("nstates", C.c_int), ("cheb_degree", C.c_int), ("rho_trigger", C.c_int),
+                ("max_scf", C.c_int), ("min_scf", C.c_int), ("tol_scf", C.c_double),
+                ("tol_poisson", C.c_double), ("maxit_poisson", C.c_int),
+                ("tol_lanczos", C.c_double), ("beta", C.c_double), ("smear_type", C.c_int),
+                ("nelec", C.c_double), ("occfac", C.c_double), ("mix_history", C.c_int),
+                ("pulay_freq", C.c_int), ("mix_beta", C.c_double), ("mix_omega", C.c_double),
+                ("kerker_ktf", C.c_double), ("kerker_thresh", C.c_double),
+                ("tol_precond", C.c_double), ("Esc", C.c_double), ("dtype", C.c_int),
+                ("psi_seed", C.c_ulonglong), ("lanczos_seed", C.c_ulonglong)]
+
+
+class SCFResult(C.Structure):
+    _fields_ = [("scf_iters", C.c_int), ("converged", C.c_int), ("Etot", C.c_double),
+                ("Eband", C.c_double), ("Exc", C.c_double), ("E1", C.c_double),
+                ("E2", C.c_double), ("E3", C.c_double), ("Entropy", C.c_double),
+                ("Efermi", C.c_double), ("scf_error", C.c_double), ("mean_iter_ms", C.c_double),
+                ("total_ms", C.c_double), ("poisson_iters", C.c_long),
+                ("kerker_iters", C.c_long)]
+
+
 RES_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_double, _dp, _dp, _dp)
 PRECOND_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_double, _dp, _dp)
 
@@ -91,6 +112,8 @@ _SIGS = {
     "sgpu_rotate_dev": (C.c_int, [_vp, _vp, _dp, _vp]),
     "sgpu_lanczos": (C.c_int, [_vp, _dp, C.c_double, C.c_double, C.c_int, _dp, _dp, _ip]),
     "sgpu_chefsi_dev": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_double, C.c_double, C.c_double, _dp]),
+    "sgpu_scf_run": (C.c_int, [_vp, C.POINTER(SCFParams), _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                               C.POINTER(SCFResult)]),
     "sgpu_timer_get": (C.c_int, [_vp, C.c_char_p, _dp, _ip]),
     "sgpu_timer_reset": (C.c_int, [_vp]),
     "sgpu_timer_enable": (C.c_int, [_vp, C.c_int]),
@@ -355,6 +378,28 @@ class Context:
         lam = np.zeros(X.ncol)
         self._check(self.lib.sgpu_chefsi_dev(self.h, X.h, Y.h, m, a, b, a0, _ptr(lam)))
         return lam
+
+    # -- SCF
+    def scf_run(self, b, rho0, params: dict):
+        """Density-mixing SCF (sgpu_scf_run).  Returns a dict with the energies,
+        eigenvalues, final density / potential and per-iteration history."""
+        p = SCFParams(**params)
+        bb = _f64(b).ravel()
+        r0 = _f64(rho0).ravel()
+        ns, mx = p.nstates, p.max_scf
+        lam = np.zeros(ns)
+        rho = np.zeros(self.nd)
+        phi = np.zeros(self.nd)
+        ims, ierr, iet = np.zeros(mx), np.zeros(mx), np.zeros(mx)
+        res = SCFResult()
+        st = self._check(self.lib.sgpu_scf_run(self.h, C.byref(p), _ptr(bb), _ptr(r0), _ptr(lam),
+                                               _ptr(rho), _ptr(phi), _ptr(ims), _ptr(ierr),
+                                               _ptr(iet), C.byref(res)))
+        k = res.scf_iters
+        out = {f: getattr(res, f) for f, _ in SCFResult._fields_}
+        out.update(lambda_=lam, rho=rho, phi=phi, iter_ms=ims[:k], iter_err=ierr[:k],
+                   iter_etot=iet[:k], status=st)
+        return out
 
     # -- timers
     def timer_enable(self, on: bool = True):
