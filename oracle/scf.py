@@ -71,7 +71,11 @@ class OracleSCF:
         rhs = 4.0 * math.pi * (self.b + rho)
         self.phi, it, _ = self.o.poisson_solve(self.phi, rhs, p["tol_poisson"], p["maxit_poisson"])
         self.poisson_iters += it
-        self.exc, self.vxc = lda_pz(rho)
+        # add_rho_core: XC evaluated at max(rho, xc_rhotol = 1e-14)
+        # (src/exchangeCorrelation.c:1254-1270, src/initialization.c:1843)
+        rxc = np.maximum(rho, 1e-14)
+        exc, self.vxc = lda_pz(rxc)
+        self.exc_density = rxc * exc
         self.o.veff = self.phi + self.vxc
 
     def chefsi(self, count: int):
@@ -109,7 +113,7 @@ class OracleSCF:
         rho_out = density(self.X, self.occ, p["occfac"], dV)
         rho_in = self.rho_in
         eband = float(np.sum(p["occfac"] * self.occ * self.lam))
-        exc = float(rho_in @ self.exc) * dV
+        exc = float(self.exc_density.sum()) * dV
         e1 = 0.5 * float(self.b @ self.phi) * dV
         e2 = 0.5 * float(rho_in @ self.phi) * dV
         e3 = float(rho_in @ self.vxc) * dV

@@ -3,6 +3,7 @@
 // pass, plus the host-pointer drop-ins of the reference entry points.
 #include <rocsolver/rocsolver.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -84,8 +85,9 @@ int filter_raw(sgpu_context *ctx, int dtype, int ncol, void *bx, void *by, long 
     return SGPU_OK;
 }
 
-// Sum a [nslice][n*n] FP32 partial stack into an FP64 n*n matrix.
-__global__ void sum_partials(const float *__restrict__ part, int nslice, long nn,
+// Sum a [nslice][n*n] partial stack into an FP64 n*n matrix.
+template <typename P>
+__global__ void sum_partials(const P *__restrict__ part, int nslice, long nn,
                              double *__restrict__ out)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -95,43 +97,79 @@ __global__ void sum_partials(const float *__restrict__ part, int nslice, long nn
     out[t] = s;
 }
 
-// C (n x n, FP64, device) = A' B over nd rows; split-K in FP32 for SGPU_F32.
-int gram_tn(sgpu_context *ctx, int dtype, int n, long nd, const void *A, long lda, const void *B,
-            long ldb, double *C)
+// Rows per K-slice of the split-K Gram product.  A TN product with n = Ns
+// and K = Nd has only (n/64)^2 output tiles, far too few for 256 CUs when
+// Ns is small (rocBLAS ran 140k x 160 in FP64 at under 1 TFLOP/s unsplit),
+// so K is cut into slices until there are about 256 (FP64) or 2048 (FP32)
+// tile-slices; more FP64 slices slowed the 474k x 480 case down.  FP32
+// slices are capped at 8192 rows: FP32 accumulation stays inside a slice and
+// the slices are summed in FP64.
+long gram_kslice(int n, long nd, int dtype)
 {
-    if (dtype == SGPU_F64) {
-        const double one = 1.0, zero = 0.0;
-        SGPU_BLAS(rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host));
-        SGPU_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, n, n,
-                                (rocblas_int)nd, &one, (const double *)A, (rocblas_int)lda,
-                                (const double *)B, (rocblas_int)ldb, &zero, C, n));
-        return SGPU_OK;
-    }
-    // FP32 products with FP32 accumulation inside K-slices of kslice rows and
-    // FP64 accumulation across slices
-    const long kslice = 8192;
+    const long tiles = std::max(1L, ((long)(n + 63) / 64) * ((n + 63) / 64));
+    const long target = dtype == SGPU_F32 ? 2048 : 256;
+    long nslice = std::max(1L, (target + tiles - 1) / tiles);
+    long ks = (nd + nslice - 1) / nslice;
+    ks = std::max(1024L, (ks + 255) / 256 * 256);
+    if (dtype == SGPU_F32) ks = std::min(ks, 8192L);
+    return std::min(ks, std::max(nd, 1L));
+}
+
+template <typename T>
+rocblas_status gemm_tn_batched(rocblas_handle h, int n, long k, const T *A, long lda, long sa,
+                               const T *B, long ldb, long sb, T *C, long sc, int batch);
+template <>
+rocblas_status gemm_tn_batched<double>(rocblas_handle h, int n, long k, const double *A, long lda,
+                                       long sa, const double *B, long ldb, long sb, double *C,
+                                       long sc, int batch)
+{
+    const double one = 1.0, zero = 0.0;
+    return rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, n,
+                                         n, (rocblas_int)k, &one, A, (rocblas_int)lda, sa, B,
+                                         (rocblas_int)ldb, sb, &zero, C, n, sc, batch);
+}
+template <>
+rocblas_status gemm_tn_batched<float>(rocblas_handle h, int n, long k, const float *A, long lda,
+                                      long sa, const float *B, long ldb, long sb, float *C,
+                                      long sc, int batch)
+{
+    const float one = 1.f, zero = 0.f;
+    return rocblas_sgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, n,
+                                         n, (rocblas_int)k, &one, A, (rocblas_int)lda, sa, B,
+                                         (rocblas_int)ldb, sb, &zero, C, n, sc, batch);
+}
+
+template <typename T>
+int gram_tn_typed(sgpu_context *ctx, int n, long nd, const T *A, long lda, const T *B, long ldb,
+                  double *C)
+{
+    const long kslice = gram_kslice(n, nd, sizeof(T) == 4 ? SGPU_F32 : SGPU_F64);
     const int nslice = (int)((nd + kslice - 1) / kslice);
     const long nn = (long)n * n;
     void *buf = nullptr;
-    SGPU_CHECK(sgpu_scratch(ctx, (size_t)nslice * nn * sizeof(float), &buf));
-    float *part = (float *)buf;
-    const float one = 1.f, zero = 0.f;
+    SGPU_CHECK(sgpu_scratch(ctx, (size_t)nslice * nn * sizeof(T), &buf));
+    T *part = (T *)buf;
     SGPU_BLAS(rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host));
     const long full = nd / kslice;
     if (full > 0)
-        SGPU_BLAS(rocblas_sgemm_strided_batched(
-            ctx->blas, rocblas_operation_transpose, rocblas_operation_none, n, n,
-            (rocblas_int)kslice, &one, (const float *)A, (rocblas_int)lda, kslice,
-            (const float *)B, (rocblas_int)ldb, kslice, &zero, part, n, nn, (rocblas_int)full));
+        SGPU_BLAS(gemm_tn_batched<T>(ctx->blas, n, kslice, A, lda, kslice, B, ldb, kslice, part, nn,
+                                     (int)full));
     const long rem = nd - full * kslice;
     if (rem > 0)
-        SGPU_BLAS(rocblas_sgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, n, n,
-                                (rocblas_int)rem, &one, (const float *)A + full * kslice,
-                                (rocblas_int)lda, (const float *)B + full * kslice,
-                                (rocblas_int)ldb, &zero, part + full * nn, n));
-    sum_partials<<<(unsigned)((nn + 255) / 256), 256, 0, ctx->stream>>>(part, nslice, nn, C);
+        SGPU_BLAS(gemm_tn_batched<T>(ctx->blas, n, rem, A + full * kslice, lda, 0,
+                                     B + full * kslice, ldb, 0, part + full * nn, nn, 1));
+    sum_partials<T><<<(unsigned)((nn + 255) / 256), 256, 0, ctx->stream>>>(part, nslice, nn, C);
     SGPU_HIP(hipGetLastError());
     return SGPU_OK;
+}
+
+// C (n x n, FP64, device) = A' B over nd rows, split-K (FP64 sums of slices).
+int gram_tn(sgpu_context *ctx, int dtype, int n, long nd, const void *A, long lda, const void *B,
+            long ldb, double *C)
+{
+    if (dtype == SGPU_F64)
+        return gram_tn_typed<double>(ctx, n, nd, (const double *)A, lda, (const double *)B, ldb, C);
+    return gram_tn_typed<float>(ctx, n, nd, (const float *)A, lda, (const float *)B, ldb, C);
 }
 
 int project_raw(sgpu_context *ctx, int dtype, int ncol, const void *Y, void *HY, long ld,

@@ -39,18 +39,21 @@ __global__ void density_kernel(const T *__restrict__ psi, long ld, int ns,
 
 // LDA: Slater exchange + Perdew-Zunger 1981 correlation (SPARC slater / pz,
 // src/exchangeCorrelation.c:459-529; J. P. Perdew and A. Zunger, PRB 23,
-// 5048 (1981), Ceperley-Alder parametrisation).
-__global__ void lda_pz_kernel(const double *__restrict__ rho, long n, double *__restrict__ exc,
-                              double *__restrict__ vxc)
+// 5048 (1981), Ceperley-Alder parametrisation), evaluated at
+// rho_xc = max(rho, 1e-14) as add_rho_core does (src/exchangeCorrelation.c:1254-1270).
+// Writes the energy density rho_xc * e_xc (Calculate_Exc, :429-454) and v_xc.
+__global__ void lda_pz_kernel(const double *__restrict__ rho_in, long n,
+                              double *__restrict__ exc_density, double *__restrict__ vxc)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const double r = fmax(rho_in[i], 1e-14);
     const double C2 = 0.738558766382022406;   // 3/4 (3/pi)^(1/3)
     const double C3 = 0.984745021842697;      // (3/pi)^(1/3)
     const double C31 = 0.620350490899400087;  // (3/(4 pi))^(1/3)
     const double A = 0.0311, B = -0.048, C = 0.002, D = -0.0116;
     const double g1 = -0.1423, b1 = 1.0529, b2 = 0.3334;
-    const double rc = cbrt(rho[i]);
+    const double rc = cbrt(r);
     const double ex = -C2 * rc, vx = -C3 * rc;
     const double rs = C31 / rc;
     double ec, vc;
@@ -63,7 +66,7 @@ __global__ void lda_pz_kernel(const double *__restrict__ rho, long n, double *__
         ec = g1 / den;
         vc = (g1 + (7.0 / 6.0) * g1 * b1 * sq + (4.0 / 3.0) * g1 * b2 * rs) / (den * den);
     }
-    exc[i] = ex + ec;
+    exc_density[i] = r * (ex + ec);
     vxc[i] = vx + vc;
 }
 
@@ -271,9 +274,12 @@ extern "C" int sgpu_scf_run(sgpu_context *ctx, const sgpu_scf_params *par, const
         SGPU_HIP(hipGetLastError());
         int pit = 0;
         double prr = 0.0;
+        sgpu_tmark tp;
+        SGPU_CHECK(sgpu_tbegin(ctx, &tp));
         int st = sgpu_aar_device(ctx, nullptr, nullptr, nullptr, 0.0, (int)nd, phi, vec[RHS], 0.6,
                                  0.6, 7, 6, par->tol_poisson, par->maxit_poisson, &pit, &prr);
         if (st < 0) return st;
+        SGPU_CHECK(sgpu_tend(ctx, &tp, "scf_poisson"));
         res->poisson_iters += pit;
         double sum = 0.0;
         const double *pa[1] = {phi}, *pb[1] = {vec[ONES]};
@@ -315,8 +321,11 @@ extern "C" int sgpu_scf_run(sgpu_context *ctx, const sgpu_scf_params *par, const
                 // Chebyshevfilter_constants (src/eigenSolver.c:589-715)
                 if (count == 0) {
                     int lit = 0;
+                    sgpu_tmark tl;
+                    SGPU_CHECK(sgpu_tbegin(ctx, &tl));
                     SGPU_CHECK(sgpu_lanczos(ctx, x0.data(), par->tol_lanczos, par->tol_lanczos, 1000,
                                             &eigmin, &eigmax, &lit));
+                    SGPU_CHECK(sgpu_tend(ctx, &tl, "scf_lanczos"));
                     eigmax *= 1.01;
                     eigmin -= 0.1;
                     cutoff = 0.5 * (eigmin + eigmax);
@@ -324,8 +333,11 @@ extern "C" int sgpu_scf_run(sgpu_context *ctx, const sgpu_scf_params *par, const
                     if (count >= par->rho_trigger) eigmin = lam[0];
                     cutoff = lam[ns - 1] + 0.1;
                 }
+                sgpu_tmark tc;
+                SGPU_CHECK(sgpu_tbegin(ctx, &tc));
                 SGPU_CHECK(sgpu_chefsi_dev(ctx, X, Y, par->cheb_degree, cutoff, eigmax, eigmin,
                                            lam.data()));
+                SGPU_CHECK(sgpu_tend(ctx, &tc, "scf_chefsi"));
                 // Calculate_occupation(eigmin - 1, eigmax + 1, 1e-12, 100)
                 if (fermi_level(lam, par->occfac, par->nelec, par->beta, par->smear_type, lam[0] - 1.0,
                                 lam[ns - 1] + 1.0, 1e-12, 100, &ef) != 0) {
@@ -335,6 +347,8 @@ extern "C" int sgpu_scf_run(sgpu_context *ctx, const sgpu_scf_params *par, const
                 for (int n = 0; n < ns; n++) occ[n] = smearing(par->beta, lam[n], ef, par->smear_type);
             }
             // ---- rho_out = occfac sum_n g_n psi_n^2 / dV
+            sgpu_tmark td;
+            SGPU_CHECK(sgpu_tbegin(ctx, &td));
             for (int n = 0; n < ns; n++) wh[n] = par->occfac * occ[n] / dV;
             SGPU_HIP(hipMemcpyAsync(w_occ, wh.data(), ns * sizeof(double), hipMemcpyHostToDevice, s));
             if (X->dtype == SGPU_F64)
@@ -346,8 +360,8 @@ extern "C" int sgpu_scf_run(sgpu_context *ctx, const sgpu_scf_params *par, const
             SGPU_HIP(hipGetLastError());
             // ---- free energy with rho_in (Calculate_Free_Energy, src/energy.c:41-199)
             double d[6];
-            const double *pa[6] = {rho_in, vec[BCH], rho_in, rho_in, rho_out, rho_out};
-            const double *pb[6] = {vec[EXC], phi, phi, vec[VXC], rho_out, rho_in};
+            const double *pa[6] = {vec[EXC], vec[BCH], rho_in, rho_in, rho_out, rho_out};
+            const double *pb[6] = {vec[ONES], phi, phi, vec[VXC], rho_out, rho_in};
             SGPU_CHECK(sgpu_dot_many(ctx, (int)nd, 6, pa, pb, d));
             double eband = 0.0;
             for (int n = 0; n < ns; n++) eband += par->occfac * occ[n] * lam[n];
@@ -363,6 +377,7 @@ extern "C" int sgpu_scf_run(sgpu_context *ctx, const sgpu_scf_params *par, const
                 SGPU_CHECK(sgpu_dot_many(ctx, (int)nd, 1, qa, qb, &dd));
             }
             const double err = std::sqrt(dd / d[4]);
+            SGPU_CHECK(sgpu_tend(ctx, &td, "scf_density_energy"));
             res->Etot = etot; res->Eband = eband; res->Exc = exc; res->E1 = e1; res->E2 = e2;
             res->E3 = e3; res->Entropy = ent; res->Efermi = ef; res->scf_error = err;
             if (!(etot == etot)) {
@@ -389,6 +404,8 @@ extern "C" int sgpu_scf_run(sgpu_context *ctx, const sgpu_scf_params *par, const
             // Kerker_precond (src/mixing.c:462-527): Lf = (L - kTF^2 idiemac) f,
             // solve -(L - kTF^2) Pf = Lf by AAR warm-started from the last Pf, Pf *= -amix
             const double k2 = par->kerker_ktf * par->kerker_ktf;
+            sgpu_tmark tk;
+            SGPU_CHECK(sgpu_tbegin(ctx, &tk));
             SGPU_CHECK(sgpu_launch_stencil(ctx, SGPU_F64, 1, 1.0, 0.0, -k2 * par->kerker_thresh,
                                            nullptr, vec[FW], ld, vec[LF], ld, nullptr, 0, 1.0, 0.0,
                                            true));
@@ -405,6 +422,7 @@ extern "C" int sgpu_scf_run(sgpu_context *ctx, const sgpu_scf_params *par, const
                 axpbz<<<nb, 256, 0, s>>>(vec[PF], 1.0, vec[PF], -sum / (double)nd, vec[ONES], nd);
             }
             axpbz<<<nb, 256, 0, s>>>(vec[PF], -amix, vec[PF], 0.0, nullptr, nd);
+            SGPU_CHECK(sgpu_tend(ctx, &tk, "scf_kerker"));
             // x_kp1 = x_wavg + Pf into the rho_out buffer, clipped at zero and
             // rescaled to the electron count
             axpbz<<<nb, 256, 0, s>>>(rho_out, 1.0, vec[XW], 1.0, vec[PF], nd);

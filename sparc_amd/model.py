@@ -62,28 +62,91 @@ class ModelSystem:
         return len(self.positions)
 
 
+def merge_nonlocal(sets) -> NonlocalSet:
+    """Concatenate projector sets (offsets shifted, alpha rows renumbered)."""
+    sets = [q for q in sets if q is not None and q.n_influence > 0]
+    po = co = ao = 0
+    parts = {k: [] for k in ("ndc", "nproj", "pos_off", "chi_off", "alpha_off", "grid_pos", "chi",
+                             "gamma")}
+    for q in sets:
+        parts["ndc"].append(q.ndc); parts["nproj"].append(q.nproj)
+        parts["pos_off"].append(q.pos_off + po); parts["chi_off"].append(q.chi_off + co)
+        parts["alpha_off"].append(q.alpha_off + ao)
+        parts["grid_pos"].append(q.grid_pos); parts["chi"].append(q.chi)
+        parts["gamma"].append(q.gamma)
+        po += q.grid_pos.size; co += q.chi.size; ao += q.n_alpha
+    cat = {k: np.concatenate(v) for k, v in parts.items()}
+    return NonlocalSet(ndc=cat["ndc"].astype(np.int32), nproj=cat["nproj"].astype(np.int32),
+                       pos_off=cat["pos_off"].astype(np.int64),
+                       grid_pos=cat["grid_pos"].astype(np.int32),
+                       chi_off=cat["chi_off"].astype(np.int64), chi=cat["chi"],
+                       alpha_off=cat["alpha_off"].astype(np.int32), n_alpha=ao,
+                       gamma=cat["gamma"], dV=sets[0].dV)
+
+
+# Per-species model parameters: valence charge, pseudocharge width, initial
+# density width, projector width and KB energies (l = 0, 1 share them).
+SPECIES = {
+    "Si": dict(zval=4.0, sigma_b=0.75, sigma_rho=1.2, proj_sigma=1.0, gammas=(2.0,)),
+    "Al": dict(zval=3.0, sigma_b=0.85, sigma_rho=1.3, proj_sigma=1.1, gammas=(1.5,)),
+}
+
+
 def silicon_model(ncell: int = 1, n: int = 26, nstates: int | None = None, a: float = 10.26,
-                  disp: float = 0.02, seed: int = 0, zval: float = 4.0, sigma_b: float = 0.75,
-                  sigma_rho: float = 1.2, proj_sigma: float = 1.0, proj_rc: float = 3.0,
-                  gammas=(2.0,), lmax: int = 1) -> ModelSystem:
-    """Diamond-cubic ncell^3 silicon-like model on an n^3 periodic grid."""
+                  disp: float = 0.02, seed: int = 0, al_frac: float = 0.0, proj_rc: float = 3.0,
+                  lmax: int = 1) -> ModelSystem:
+    """Diamond-cubic ncell^3 silicon-like model on an n^3 periodic grid, with
+    round(al_frac * n_atom) seeded Al substitutions."""
     L = a * ncell
     h = L / n
     g = Grid(n, n, n, h, h, h)
     pos = diamond_cubic(ncell, a, disp, seed) % L
     nat = len(pos)
-    nelec = zval * nat
-    b = gaussian_density(g, pos, -zval, sigma_b)
+    kind = np.array(["Si"] * nat, dtype=object)
+    n_al = int(round(al_frac * nat))
+    if n_al:
+        kind[np.random.default_rng(seed + 1).choice(nat, n_al, replace=False)] = "Al"
+    b = np.zeros(g.nd)
+    rho0 = np.zeros(g.nd)
+    esc, nelec, nls = 0.0, 0.0, []
+    for sp, prm in SPECIES.items():
+        sel = pos[kind == sp]
+        if len(sel) == 0:
+            continue
+        z = prm["zval"]
+        nelec += z * len(sel)
+        b += gaussian_density(g, sel, -z, prm["sigma_b"])
+        rho0 += gaussian_density(g, sel, z, prm["sigma_rho"])
+        esc -= len(sel) * z * z / (2.0 * np.sqrt(np.pi) * prm["sigma_b"])
+        nls.append(gaussian_projectors(g, sel, min(proj_rc, 0.49 * L),
+                                       sigmas=(prm["proj_sigma"],) * len(prm["gammas"]),
+                                       gammas=prm["gammas"], lmax=lmax))
     b *= -nelec / (b.sum() * g.dV)
-    rho0 = gaussian_density(g, pos, zval, sigma_rho)
     rho0 *= nelec / (rho0.sum() * g.dV)
-    esc = -nat * zval * zval / (2.0 * np.sqrt(np.pi) * sigma_b)
-    nl = gaussian_projectors(g, pos, min(proj_rc, 0.49 * L), sigmas=(proj_sigma,) * len(gammas),
-                             gammas=gammas, lmax=lmax)
     if nstates is None:
-        nstates = int(nelec / 2 * 1.2 + 5)  # SPARC's default Nstates (src/initialization.c:1716)
-    return ModelSystem(g, pos, zval, b, rho0, esc, nl, nelec, nstates,
-                       meta={"ncell": ncell, "n": n, "a": a, "h": h})
+        nstates = int(int(nelec) // 2 * 1.2 + 5)  # SPARC's default Nstates (src/initialization.c:1716)
+    return ModelSystem(g, pos, float(SPECIES["Si"]["zval"]), b, rho0, esc, merge_nonlocal(nls),
+                       nelec, nstates,
+                       meta={"ncell": ncell, "n": n, "a": a, "h": h, "n_al": n_al,
+                             "species": list(kind)})
+
+
+def scf_config(name: str) -> tuple[ModelSystem, dict]:
+    """The SCF workloads by name: 'si8' (26^3, 24 states, Gaussian smearing
+    0.2 eV), 'si64' (52^3, 160 states) and 'sial216' (78^3, 480 states, 10 %
+    Al, 0.03 Bohr displacements, Fermi-Dirac smearing 0.01 Ha)."""
+    if name == "si8":
+        sysm = silicon_model(1, 26, 24)
+        return sysm, default_params(sysm)
+    if name == "si64":
+        sysm = silicon_model(2, 52, 160)
+        return sysm, default_params(sysm)
+    if name == "sial216":
+        sysm = silicon_model(3, 78, 480, disp=0.03, al_frac=0.1)
+        p = default_params(sysm)
+        p.update(smear_type=0, beta=100.0)
+        return sysm, p
+    raise ValueError(f"unknown SCF workload {name!r}")
 
 
 def default_params(sysm: ModelSystem, precision: int = 0, tol_scf: float | None = None,
