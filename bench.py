@@ -14,10 +14,11 @@ Algorithmic bytes per filter:
     s*Nd*Ns*(2 + 3(m-1)) + m*s*Nd + 2*m*s*sum(Chi)        (s = 4 or 8)
 
 The SCF half of the metric is reported under "scf": full device SCF runs of
-the silicon-like model systems (Gaussian pseudocharges and projectors, LDA-PZ,
-Pulay/Kerker mixing, the SPARC defaults) in FP64 and in mixed precision, with
-the mean SCF-iteration time over iterations 2..N and the energy and eigenvalue
-differences of the mixed run against the FP64 run.
+diamond-cubic Si (64 atoms) and Si/Al (216 atoms) with the ONCV psp8
+pseudopotentials (NLCC), LDA-PZ, Pulay/Kerker mixing and SPARC's defaults, in
+FP64 and in mixed precision, with the mean SCF-iteration time over iterations
+2..N and the energy and eigenvalue differences of the mixed run against the
+FP64 run.
 """
 from __future__ import annotations
 
@@ -122,26 +123,30 @@ def cpu_baseline(wl, ncol_sample=48, degree=None):
 
 
 def scf_bench(name):
-    """FP64 and mixed-precision device SCF of one model system."""
+    """FP64 and mixed-precision device SCF of one psp8 system."""
     from sparc_amd import gpu
-    from sparc_amd.model import scf_config
+    from sparc_amd.systems import scf_system
     t0 = time.perf_counter()
-    sysm, p = scf_config(name)
-    print(f"[bench] scf {name}: {sysm.n_atom} atoms, {sysm.grid.nx}^3, {p['nstates']} states, "
+    st, p = scf_system(name)
+    g = st.grid
+    print(f"[bench] scf {name}: {st.n_atom} atoms, {g.nx}^3, {p['nstates']} states, "
           f"setup {time.perf_counter() - t0:.1f}s", flush=True)
-    out = {"workload": f"{name}: {sysm.n_atom}-atom diamond-cubic Si-like model"
-                       + (f" ({sysm.meta['n_al']} Al)" if sysm.meta["n_al"] else "")
-                       + f", {sysm.grid.nx}^3 grid h={sysm.grid.hx:.4f}, {p['nstates']} states, "
-                       f"degree {p['cheb_degree']}, LDA-PZ, "
+    n_al = st.meta.get("n_al", 0)
+    out = {"workload": f"{name}: {st.n_atom}-atom "
+                       + ("SiC (reference test input)" if name == "sic8" else
+                          "diamond-cubic Si" + (f" with {n_al} Al" if n_al else ""))
+                       + f", ONCV psp8 with NLCC, {g.nx}^3 grid h={g.hx:.4f}, {p['nstates']} "
+                       f"states, degree {p['cheb_degree']}, "
+                       + ("LDA_PW" if p["xc"] == 1 else "LDA_PZ") + ", "
                        + ("Fermi-Dirac" if p["smear_type"] == 0 else "Gaussian")
-                       + f" smearing beta={p['beta']:.2f}, tol_scf {p['tol_scf']:.3g}"}
+                       + f" smearing {1.0 / p['beta']:.4g} Ha, tol_scf {p['tol_scf']:.3g}"}
     runs = {}
-    with gpu.Context(sysm.grid) as ctx:
-        ctx.set_nonlocal(sysm.nloc)
+    with gpu.Context(g) as ctx:
+        ctx.set_nonlocal(st.nloc)
         for tag, dt in (("f64", gpu.F64), ("mixed", gpu.F32)):
             ctx.timer_reset()
             ctx.timer_enable(True)
-            r = ctx.scf_run(sysm.b, sysm.rho0, dict(p, dtype=dt))
+            r = ctx.scf_run(st.b, st.rho_at, dict(p, dtype=dt), rho_core=st.rho_core)
             ctx.timer_enable(False)
             phases = {}
             for k in ("scf_lanczos", "scf_chefsi", "scf_poisson", "scf_kerker",
@@ -154,36 +159,38 @@ def scf_bench(name):
             out[tag] = {"ms_per_scf_iter": round(r["mean_iter_ms"], 3),
                         "total_ms": round(r["total_ms"], 1), "scf_iters": r["scf_iters"],
                         "converged": bool(r["converged"]), "Etot": r["Etot"],
-                        "Etot_per_atom": r["Etot"] / sysm.n_atom,
+                        "Etot_per_atom": r["Etot"] / st.n_atom,
                         "phase_ms_per_iter": phases}
             print(f"[bench] scf {name} {tag}: {r['scf_iters']} iters, "
-                  f"{r['mean_iter_ms']:.2f} ms/iter, Etot {r['Etot']:.8f}", flush=True)
+                  f"{r['mean_iter_ms']:.2f} ms/iter, E/atom {r['Etot'] / st.n_atom:.10f}",
+                  flush=True)
     f, m = runs["f64"], runs["mixed"]
-    out["dE_per_atom_mixed_vs_f64"] = abs(m["Etot"] - f["Etot"]) / sysm.n_atom
+    out["dE_per_atom_mixed_vs_f64"] = abs(m["Etot"] - f["Etot"]) / st.n_atom
     out["max_dlambda_mixed_vs_f64"] = float(np.max(np.abs(m["lambda_"] - f["lambda_"])))
     out["speedup_mixed_vs_f64"] = round(f["mean_iter_ms"] / m["mean_iter_ms"], 3)
     return out
 
 
-def cpu_baseline_scf():
-    """The FP64 oracle SCF of the 8-atom model on the host cores (whole run)."""
+def cpu_baseline_scf(name="si8"):
+    """The FP64 oracle SCF of the 8-atom Si cell on the host cores (whole run)."""
     from oracle import oracle as O
     from oracle.scf import OracleSCF
-    from sparc_amd.model import scf_config
+    from sparc_amd.systems import scf_system
     from sparc_amd.workloads import uniform_block
     threads = min(16, os.cpu_count() or 1)
     O.set_threads(threads)
-    sysm, p = scf_config("si8")
-    nd = sysm.grid.nd
+    st, p = scf_system(name)
+    nd = st.grid.nd
     psi0 = uniform_block(nd, p["nstates"], p["psi_seed"], -0.5, 0.5)
     x0 = uniform_block(nd, 1, p["lanczos_seed"], 0.0, 1.0)
     t0 = time.perf_counter()
-    r = OracleSCF(sysm.grid, sysm.nloc, sysm.b, sysm.rho0, p, psi0, x0).run()
+    r = OracleSCF(st.grid, st.nloc, st.b, st.rho_at, p, psi0, x0, rho_core=st.rho_core).run()
     dt = time.perf_counter() - t0
     return {"value": round(dt * 1e3 / r["scf_iters"], 2), "unit": "ms/SCF-iteration",
             "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle SCF, si8 model, whole run: {r['scf_iters']} iterations in "
-                      f"{dt:.2f} s (mean includes the first iteration)"}
+            "sample": f"FP64 oracle SCF, {name} (8-atom Si, psp8, 26^3, 24 states), whole run: "
+                      f"{r['scf_iters']} iterations in {dt:.2f} s (mean includes the first "
+                      f"iteration)"}
 
 
 def main(argv=None):
@@ -197,7 +204,8 @@ def main(argv=None):
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--scf", default="si64,sial216",
-                    help="comma-separated SCF workloads to time (si8, si64, sial216) or 'none'")
+                    help="comma-separated SCF workloads to time (sic8, si8, si64, sial216) or "
+                         "'none'")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one device copy of a 512-vector FP32 block (known bytes) so "
                          "that a profile holds a plain copy kernel to scale HBM counters by")

@@ -177,6 +177,10 @@ int sgpu_chefsi_dev(sgpu_context *ctx, sgpu_block *X, sgpu_block *Y, int m, doub
 
 /* ---- self-consistent field loop ------------------------------------- */
 
+/* Exchange-correlation functionals (EXCHANGE_CORRELATION: LDA_PZ / LDA_PW) */
+#define SGPU_XC_LDA_PZ 0
+#define SGPU_XC_LDA_PW 1
+
 /* Parameters of the density-mixing SCF (defaults of SPARC in brackets,
  * src/initialization.c:677-989,2233-2608). */
 typedef struct {
@@ -201,6 +205,7 @@ typedef struct {
     double kerker_thresh;   /* precond_kerker_thresh [0.1] */
     double tol_precond;     /* TOL_PRECOND [h^2 1e-3] */
     double Esc;             /* self + correction energy of the pseudocharges */
+    int xc;                 /* SGPU_XC_LDA_PZ or SGPU_XC_LDA_PW */
     int dtype;              /* SGPU_F64 or SGPU_F32 vector blocks (filter, Gram, rotation) */
     unsigned long long psi_seed;      /* initial orbitals uniform [-0.5, 0.5) */
     unsigned long long lanczos_seed;  /* Lanczos start vector uniform [0, 1) */
@@ -219,13 +224,16 @@ typedef struct {
 /* Density-mixing SCF with CheFSI: replaces scf / scf_loop
  * (src/electronicGroundState.c:607-1147) for one spin-unpolarized Gamma-point
  * system whose grid, Veff storage and nonlocal projectors live in ctx.  b is
- * the pseudocharge density and rho0 the initial electron density (Nd each,
- * host).  Outputs: lambda (nstates eigenvalues of the last pass), optional
+ * the pseudocharge density, rho0 the initial electron density and rho_core
+ * the NLCC model core density added to rho for exchange-correlation
+ * (add_rho_core, src/exchangeCorrelation.c:1254-1270; may be NULL), Nd each,
+ * host.  Outputs: lambda (nstates eigenvalues of the last pass), optional
  * final mixed density / electrostatic potential (Nd each) and per-iteration
  * wall time, SCF error and total energy (max_scf entries each).  Returns
  * SGPU_NOT_CONVERGED if max_scf was reached. */
 int sgpu_scf_run(sgpu_context *ctx, const sgpu_scf_params *par, const double *b,
-                 const double *rho0, double *lambda, double *rho_out, double *phi_out,
+                 const double *rho0, const double *rho_core, double *lambda, double *rho_out,
+                 double *phi_out,
                  double *iter_ms, double *iter_err, double *iter_etot, sgpu_scf_result *res);
 
 /* ---- timing ---------------------------------------------------------- */

@@ -93,6 +93,7 @@ def lib():
         L.orc_fermi_level.restype = C.c_double
         L.orc_density.argtypes = [C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_double, C.c_double, _dp]
         L.orc_lda_pz.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.orc_lda_pw.argtypes = [C.c_int, _dp, _dp, _dp]
         L.orc_set_threads.argtypes = [C.c_int]
         _lib = L
     return _lib
@@ -294,6 +295,18 @@ def lda_pz(rho):
     exc, vxc = np.zeros_like(rho), np.zeros_like(rho)
     lib().orc_lda_pz(rho.size, _p(rho), _p(exc), _p(vxc))
     return exc, vxc
+
+
+def lda_pw(rho):
+    rho = np.ascontiguousarray(rho, np.float64)
+    exc, vxc = np.zeros_like(rho), np.zeros_like(rho)
+    lib().orc_lda_pw(rho.size, _p(rho), _p(exc), _p(vxc))
+    return exc, vxc
+
+
+def lda(rho, xc: int = 0):
+    """xc 0: LDA_PZ, 1: LDA_PW."""
+    return lda_pw(rho) if xc == 1 else lda_pz(rho)
 
 
 # ------------------------------------------------------------ closed forms

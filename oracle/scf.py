@@ -17,7 +17,7 @@ import math
 
 import numpy as np
 
-from .oracle import Oracle, anderson_coeff, density, fermi_level, lda_pz, smearing, sygv
+from .oracle import Oracle, anderson_coeff, density, fermi_level, lda, smearing, sygv
 
 
 def entropy(lam, occ, ef, beta, smear_type, occfac):
@@ -38,8 +38,10 @@ def entropy(lam, occ, ef, beta, smear_type, occfac):
 class OracleSCF:
     """State of one SCF run; ``run()`` iterates to convergence."""
 
-    def __init__(self, grid, nloc, b, rho0, params: dict, psi0: np.ndarray, x0: np.ndarray):
+    def __init__(self, grid, nloc, b, rho0, params: dict, psi0: np.ndarray, x0: np.ndarray,
+                 rho_core=None):
         self.g = grid
+        self.rho_core = None if rho_core is None else np.ascontiguousarray(rho_core, np.float64).ravel()
         self.p = dict(params)
         self.o = Oracle(grid, None, nloc)
         nd = grid.nd
@@ -71,10 +73,10 @@ class OracleSCF:
         rhs = 4.0 * math.pi * (self.b + rho)
         self.phi, it, _ = self.o.poisson_solve(self.phi, rhs, p["tol_poisson"], p["maxit_poisson"])
         self.poisson_iters += it
-        # add_rho_core: XC evaluated at max(rho, xc_rhotol = 1e-14)
+        # add_rho_core: XC evaluated at max(rho + rho_core, xc_rhotol = 1e-14)
         # (src/exchangeCorrelation.c:1254-1270, src/initialization.c:1843)
-        rxc = np.maximum(rho, 1e-14)
-        exc, self.vxc = lda_pz(rxc)
+        rxc = np.maximum(rho if self.rho_core is None else rho + self.rho_core, 1e-14)
+        exc, self.vxc = lda(rxc, p.get("xc", 0))
         self.exc_density = rxc * exc
         self.o.veff = self.phi + self.vxc
 

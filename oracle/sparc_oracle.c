@@ -673,3 +673,27 @@ void orc_lda_pz(int n, const double *rho, double *exc, double *vxc)
         vxc[i] = vx + vc;
     }
 }
+
+/* Slater exchange + Perdew-Wang 1992 correlation, p = 1 parameter set
+ * (J. P. Perdew and Y. Wang, PRB 45, 13244 (1992), eq. 10; SPARC pw,
+ * src/exchangeCorrelation.c:475-499).  v_c = e_c - (rs/3) de_c/drs. */
+void orc_lda_pw(int n, const double *rho, double *exc, double *vxc)
+{
+    const double C2 = 0.75 * cbrt(3.0 / M_PI), C3 = cbrt(3.0 / M_PI);
+    const double C31 = cbrt(3.0 / (4.0 * M_PI));
+    const double A = 0.031091, a1 = 0.21370;
+    const double b1 = 7.5957, b2 = 3.5876, b3 = 1.6382, b4 = 0.49294;
+    for (int i = 0; i < n; i++) {
+        double rc = cbrt(rho[i]);
+        double ex = -C2 * rc, vx = -C3 * rc;
+        double rs = C31 / rc, sq = sqrt(rs);
+        double Q = b1 * sq + b2 * rs + b3 * rs * sq + b4 * rs * rs;
+        double dQ = 0.5 * b1 / sq + b2 + 1.5 * b3 * sq + 2.0 * b4 * rs;
+        double L = log(1.0 + 1.0 / (2.0 * A * Q));
+        double ec = -2.0 * A * (1.0 + a1 * rs) * L;
+        /* d/drs ln(1 + 1/(2AQ)) = -dQ / (Q (1 + 2AQ)) */
+        double dec = -2.0 * A * a1 * L + 2.0 * A * (1.0 + a1 * rs) * dQ / (Q * (1.0 + 2.0 * A * Q));
+        exc[i] = ex + ec;
+        vxc[i] = vx + ec - rs / 3.0 * dec;
+    }
+}

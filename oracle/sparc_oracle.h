@@ -128,6 +128,8 @@ void orc_density(int nd, int ns, const double *psi, int ld, const double *occ, d
 
 /* LDA: Slater exchange + Perdew-Zunger correlation (src/exchangeCorrelation.c:459-529). */
 void orc_lda_pz(int n, const double *rho, double *exc, double *vxc);
+/* LDA: Slater exchange + Perdew-Wang 92 correlation (src/exchangeCorrelation.c:475-499). */
+void orc_lda_pw(int n, const double *rho, double *exc, double *vxc);
 
 /* Set the number of OpenMP threads used by the column-parallel loops. */
 void orc_set_threads(int n);

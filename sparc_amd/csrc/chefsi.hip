@@ -101,12 +101,14 @@ __global__ void sum_partials(const P *__restrict__ part, int nslice, long nn,
 // and K = Nd has only (n/64)^2 output tiles, far too few for 256 CUs when
 // Ns is small (rocBLAS ran 140k x 160 in FP64 at under 1 TFLOP/s unsplit),
 // so K is cut into slices until there are about 256 (FP64) or 2048 (FP32)
-// tile-slices; more FP64 slices slowed the 474k x 480 case down.  FP32
+// tile-slices.  FP64 products with n >= 384 run unsplit: rocBLAS reached
+// 45 TFLOP/s on 474k x 480 unsplit but 5-30x less with 4-32 slices.  FP32
 // slices are capped at 8192 rows: FP32 accumulation stays inside a slice and
 // the slices are summed in FP64.
 long gram_kslice(int n, long nd, int dtype)
 {
     const long tiles = std::max(1L, ((long)(n + 63) / 64) * ((n + 63) / 64));
+    if (dtype == SGPU_F64 && n >= 384) return std::max(nd, 1L);
     const long target = dtype == SGPU_F32 ? 2048 : 256;
     long nslice = std::max(1L, (target + tiles - 1) / tiles);
     long ks = (nd + nslice - 1) / nslice;

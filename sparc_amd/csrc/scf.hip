@@ -37,34 +37,50 @@ __global__ void density_kernel(const T *__restrict__ psi, long ld, int ns,
     rho[i] = s;
 }
 
-// LDA: Slater exchange + Perdew-Zunger 1981 correlation (SPARC slater / pz,
-// src/exchangeCorrelation.c:459-529; J. P. Perdew and A. Zunger, PRB 23,
-// 5048 (1981), Ceperley-Alder parametrisation), evaluated at
-// rho_xc = max(rho, 1e-14) as add_rho_core does (src/exchangeCorrelation.c:1254-1270).
-// Writes the energy density rho_xc * e_xc (Calculate_Exc, :429-454) and v_xc.
-__global__ void lda_pz_kernel(const double *__restrict__ rho_in, long n,
-                              double *__restrict__ exc_density, double *__restrict__ vxc)
+// LDA exchange-correlation at rho_xc = max(rho + rho_core, 1e-14), as
+// add_rho_core does (src/exchangeCorrelation.c:1254-1270): Slater exchange
+// (slater, :459-473) plus Perdew-Zunger 1981 correlation (pz, :505-529;
+// PRB 23, 5048, Ceperley-Alder fit) or Perdew-Wang 1992 correlation (pw,
+// :475-499; PRB 45, 13244, p = 1 parameter set).  Writes the energy density
+// rho_xc * e_xc (Calculate_Exc, :429-454) and v_xc.
+__global__ void lda_kernel(const double *__restrict__ rho_in, const double *__restrict__ rho_core,
+                           long n, int xc, double *__restrict__ exc_density,
+                           double *__restrict__ vxc)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double r = fmax(rho_in[i], 1e-14);
+    const double r = fmax(rho_in[i] + (rho_core ? rho_core[i] : 0.0), 1e-14);
     const double C2 = 0.738558766382022406;   // 3/4 (3/pi)^(1/3)
     const double C3 = 0.984745021842697;      // (3/pi)^(1/3)
     const double C31 = 0.620350490899400087;  // (3/(4 pi))^(1/3)
-    const double A = 0.0311, B = -0.048, C = 0.002, D = -0.0116;
-    const double g1 = -0.1423, b1 = 1.0529, b2 = 0.3334;
     const double rc = cbrt(r);
     const double ex = -C2 * rc, vx = -C3 * rc;
     const double rs = C31 / rc;
     double ec, vc;
-    if (rs < 1.0) {
-        const double lr = log(rs);
-        ec = A * lr + B + C * rs * lr + D * rs;
-        vc = lr * (A + (2.0 / 3.0) * C * rs) + (B - A / 3.0) + (2.0 * D - C) * rs / 3.0;
+    if (xc == SGPU_XC_LDA_PW) {
+        const double A = 0.031091, a1 = 0.21370;
+        const double b1 = 7.5957, b2 = 3.5876, b3 = 1.6382, b4 = 0.49294;
+        const double sq = sqrt(rs);
+        const double q = b1 * sq + b2 * rs + b3 * rs * sq + b4 * rs * rs;   // p = 1
+        const double G2 = 2.0 * A * q;
+        const double G1 = log(1.0 + 1.0 / G2);
+        ec = -2.0 * A * (1.0 + a1 * rs) * G1;
+        // d ec / d rs, with d G2 / d rs = A (b1 / sqrt(rs) + 2 b2 + 3 b3 sqrt(rs) + 4 b4 rs)
+        const double dG2 = A * (b1 / sq + 2.0 * b2 + 3.0 * b3 * sq + 4.0 * b4 * rs);
+        const double dec = -2.0 * A * a1 * G1 + 2.0 * A * (1.0 + a1 * rs) * dG2 / (G2 * (G2 + 1.0));
+        vc = ec - rs / 3.0 * dec;
     } else {
-        const double sq = sqrt(rs), den = 1.0 + b1 * sq + b2 * rs;
-        ec = g1 / den;
-        vc = (g1 + (7.0 / 6.0) * g1 * b1 * sq + (4.0 / 3.0) * g1 * b2 * rs) / (den * den);
+        const double A = 0.0311, B = -0.048, C = 0.002, D = -0.0116;
+        const double g1 = -0.1423, b1 = 1.0529, b2 = 0.3334;
+        if (rs < 1.0) {
+            const double lr = log(rs);
+            ec = A * lr + B + C * rs * lr + D * rs;
+            vc = lr * (A + (2.0 / 3.0) * C * rs) + (B - A / 3.0) + (2.0 * D - C) * rs / 3.0;
+        } else {
+            const double sq = sqrt(rs), den = 1.0 + b1 * sq + b2 * rs;
+            ec = g1 / den;
+            vc = (g1 + (7.0 / 6.0) * g1 * b1 * sq + (4.0 / 3.0) * g1 * b2 * rs) / (den * den);
+        }
     }
     exc_density[i] = r * (ex + ec);
     vxc[i] = vx + vc;
@@ -229,7 +245,8 @@ double entropy(const std::vector<double> &lam, const std::vector<double> &occ, d
 }  // namespace
 
 extern "C" int sgpu_scf_run(sgpu_context *ctx, const sgpu_scf_params *par, const double *b_host,
-                            const double *rho0_host, double *lambda_out, double *rho_out_host,
+                            const double *rho0_host, const double *rho_core_host,
+                            double *lambda_out, double *rho_out_host,
                             double *phi_out_host, double *iter_ms, double *iter_err,
                             double *iter_etot, sgpu_scf_result *res)
 {
@@ -240,7 +257,8 @@ extern "C" int sgpu_scf_run(sgpu_context *ctx, const sgpu_scf_params *par, const
     const int ns = par->nstates, m = par->mix_history;
     if (ns <= 0 || par->cheb_degree < 1 || m < 1 || m > 8 || par->max_scf < 1 || par->beta <= 0 ||
         par->nelec <= 0 || (par->dtype != SGPU_F64 && par->dtype != SGPU_F32) ||
-        par->nelec > par->occfac * ns || par->rho_trigger < 1 || par->pulay_freq < 1) {
+        par->nelec > par->occfac * ns || par->rho_trigger < 1 || par->pulay_freq < 1 ||
+        (par->xc != SGPU_XC_LDA_PZ && par->xc != SGPU_XC_LDA_PW)) {
         sgpu_set_error("sgpu_scf_run: bad parameters");
         return SGPU_ERR_ARG;
     }
@@ -252,7 +270,8 @@ extern "C" int sgpu_scf_run(sgpu_context *ctx, const sgpu_scf_params *par, const
     hipStream_t s = ctx->stream;
 
     // FP64 device vectors of stride ld, then the Pulay history and small arrays
-    enum { RHO_IN, RHO_OUT, XKM1, FK, FKM1, PF, XW, FW, PHI, VXC, EXC, BCH, RHS, LF, ONES, NVEC };
+    enum { RHO_IN, RHO_OUT, XKM1, FK, FKM1, PF, XW, FW, PHI, VXC, EXC, BCH, RHS, LF, ONES, CORE,
+           NVEC };
     double *mem = nullptr;
     const size_t elems = (size_t)(NVEC + 2 * m) * ld + ns + 64;
     SGPU_HIP(hipMalloc((void **)&mem, elems * sizeof(double)));
@@ -285,7 +304,8 @@ extern "C" int sgpu_scf_run(sgpu_context *ctx, const sgpu_scf_params *par, const
         const double *pa[1] = {phi}, *pb[1] = {vec[ONES]};
         SGPU_CHECK(sgpu_dot_many(ctx, (int)nd, 1, pa, pb, &sum));
         axpbz<<<nb, 256, 0, s>>>(phi, 1.0, phi, -sum / (double)nd, vec[ONES], nd);
-        lda_pz_kernel<<<nb, 256, 0, s>>>(rho, nd, vec[EXC], vec[VXC]);
+        lda_kernel<<<nb, 256, 0, s>>>(rho, rho_core_host ? vec[CORE] : nullptr, nd, par->xc,
+                                      vec[EXC], vec[VXC]);
         veff_kernel<<<nb, 256, 0, s>>>(phi, vec[VXC], nd, ctx->veff64, ctx->veff32);
         SGPU_HIP(hipGetLastError());
         ctx->has_veff = true;
@@ -296,6 +316,9 @@ extern "C" int sgpu_scf_run(sgpu_context *ctx, const sgpu_scf_params *par, const
         SGPU_HIP(hipMemsetAsync(mem, 0, elems * sizeof(double), s));
         fill_kernel<<<nb, 256, 0, s>>>(vec[ONES], 1.0, nd);
         SGPU_HIP(hipMemcpyAsync(vec[BCH], b_host, nd * sizeof(double), hipMemcpyHostToDevice, s));
+        if (rho_core_host)
+            SGPU_HIP(hipMemcpyAsync(vec[CORE], rho_core_host, nd * sizeof(double),
+                                    hipMemcpyHostToDevice, s));
         SGPU_HIP(hipMemcpyAsync(rho_in, rho0_host, nd * sizeof(double), hipMemcpyHostToDevice, s));
         SGPU_HIP(hipMemcpyAsync(vec[XKM1], rho0_host, nd * sizeof(double), hipMemcpyHostToDevice, s));
         SGPU_CHECK(sgpu_block_create(ctx, ns, par->dtype, &X));

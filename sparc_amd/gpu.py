@@ -55,7 +55,8 @@ class SCFParams(C.Structure):
                 ("nelec", C.c_double), ("occfac", C.c_double), ("mix_history", C.c_int),
                 ("pulay_freq", C.c_int), ("mix_beta", C.c_double), ("mix_omega", C.c_double),
                 ("kerker_ktf", C.c_double), ("kerker_thresh", C.c_double),
-                ("tol_precond", C.c_double), ("Esc", C.c_double), ("dtype", C.c_int),
+                ("tol_precond", C.c_double), ("Esc", C.c_double), ("xc", C.c_int),
+                ("dtype", C.c_int),
                 ("psi_seed", C.c_ulonglong), ("lanczos_seed", C.c_ulonglong)]
 
 
@@ -113,7 +114,7 @@ _SIGS = {
     "sgpu_lanczos": (C.c_int, [_vp, _dp, C.c_double, C.c_double, C.c_int, _dp, _dp, _ip]),
     "sgpu_chefsi_dev": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_double, C.c_double, C.c_double, _dp]),
     "sgpu_scf_run": (C.c_int, [_vp, C.POINTER(SCFParams), _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
-                               C.POINTER(SCFResult)]),
+                               _dp, C.POINTER(SCFResult)]),
     "sgpu_timer_get": (C.c_int, [_vp, C.c_char_p, _dp, _ip]),
     "sgpu_timer_reset": (C.c_int, [_vp]),
     "sgpu_timer_enable": (C.c_int, [_vp, C.c_int]),
@@ -380,19 +381,21 @@ class Context:
         return lam
 
     # -- SCF
-    def scf_run(self, b, rho0, params: dict):
+    def scf_run(self, b, rho0, params: dict, rho_core=None):
         """Density-mixing SCF (sgpu_scf_run).  Returns a dict with the energies,
         eigenvalues, final density / potential and per-iteration history."""
         p = SCFParams(**params)
         bb = _f64(b).ravel()
         r0 = _f64(rho0).ravel()
+        rc = None if rho_core is None else _f64(rho_core).ravel()
         ns, mx = p.nstates, p.max_scf
         lam = np.zeros(ns)
         rho = np.zeros(self.nd)
         phi = np.zeros(self.nd)
         ims, ierr, iet = np.zeros(mx), np.zeros(mx), np.zeros(mx)
         res = SCFResult()
-        st = self._check(self.lib.sgpu_scf_run(self.h, C.byref(p), _ptr(bb), _ptr(r0), _ptr(lam),
+        st = self._check(self.lib.sgpu_scf_run(self.h, C.byref(p), _ptr(bb), _ptr(r0), _ptr(rc),
+                                               _ptr(lam),
                                                _ptr(rho), _ptr(phi), _ptr(ims), _ptr(ierr),
                                                _ptr(iet), C.byref(res)))
         k = res.scf_iters

@@ -163,4 +163,4 @@ def default_params(sysm: ModelSystem, precision: int = 0, tol_scf: float | None 
                 tol_lanczos=1e-2, beta=1.0 / (0.2 / 27.211386245988), smear_type=1,
                 nelec=sysm.nelec, occfac=2.0, mix_history=7, pulay_freq=1, mix_beta=0.3,
                 mix_omega=0.3, kerker_ktf=1.0, kerker_thresh=0.1, tol_precond=h * h * 1e-3,
-                Esc=sysm.Esc, dtype=precision, psi_seed=1234, lanczos_seed=4321)
+                Esc=sysm.Esc, xc=0, dtype=precision, psi_seed=1234, lanczos_seed=4321)

@@ -76,3 +76,57 @@ def test_scf_rejects_bad_params(case):
         for bad in (dict(nstates=10), dict(mix_history=9), dict(dtype=7), dict(beta=-1.0)):
             with pytest.raises(gpu.SparcGPUError):
                 ctx.scf_run(sysm.b, sysm.rho0, dict(p, **bad))
+
+
+@pytest.fixture(scope="module")
+def sic():
+    import json
+    from pathlib import Path
+    from oracle import oracle as orc
+    from oracle.scf import OracleSCF
+    from sparc_amd.systems import sic8
+    from sparc_amd.workloads import uniform_block
+    gold = json.loads((Path(__file__).parent / "golden" / "sic8_refout.json").read_text())
+    orc.set_threads(16)
+    st, p = sic8()
+    nd = st.grid.nd
+    psi0 = uniform_block(nd, p["nstates"], p["psi_seed"], -0.5, 0.5)
+    x0 = uniform_block(nd, 1, p["lanczos_seed"], 0.0, 1.0)
+    ref = OracleSCF(st.grid, st.nloc, st.b, st.rho_at, p, psi0, x0, rho_core=st.rho_core).run()
+    return st, p, gold, ref
+
+
+def _run_psp(st, p):
+    from sparc_amd import gpu
+    with gpu.Context(st.grid) as ctx:
+        ctx.set_nonlocal(st.nloc)
+        return ctx.scf_run(st.b, st.rho_at, p, rho_core=st.rho_core)
+
+
+def test_scf_sic_fp64_matches_reference_and_oracle(sic):
+    """psp8 SiC (LDA_PW, NLCC): the FP64 device SCF reaches the reference's
+    printed free energy and the oracle's eigenvalues."""
+    st, p, gold, ref = sic
+    r = _run_psp(st, p)
+    e = r["Etot"] / st.n_atom
+    print(f"\nSiC FP64: {r['scf_iters']} iters, E/atom {e:.10f} (reference "
+          f"{gold['free_energy_per_atom']:.10f}, oracle {ref['Etot'] / st.n_atom:.10f})")
+    assert r["converged"] == 1
+    assert abs(e - gold["free_energy_per_atom"]) < 1e-6
+    assert abs(r["Efermi"] - gold["fermi_level"]) < 1e-6
+    assert abs(r["Exc"] - gold["exchange_correlation_energy"]) < 1e-5
+    assert r["scf_iters"] == ref["scf_iters"]
+    assert np.max(np.abs(r["lambda_"] - ref["lambda_"])) < 1e-6
+
+
+def test_scf_sic_mixed_matches_reference(sic):
+    from sparc_amd import gpu
+    st, p, gold, ref = sic
+    r = _run_psp(st, dict(p, dtype=gpu.F32))
+    de = abs(r["Etot"] / st.n_atom - gold["free_energy_per_atom"])
+    dl = np.max(np.abs(r["lambda_"] - ref["lambda_"]))
+    print(f"\nSiC mixed: {r['scf_iters']} iters, dE/atom vs reference {de:.2e}, "
+          f"max dlambda vs oracle {dl:.2e}")
+    assert r["converged"] == 1
+    assert de < 1e-5
+    assert dl < 1e-5
