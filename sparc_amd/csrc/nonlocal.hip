@@ -21,6 +21,7 @@
 // reference (overlapping spheres read the same points once per atom) turns
 // into dense zeros inside Phi_b; both are counted in the measured rates.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -244,6 +245,195 @@ __global__ __launch_bounds__(256) void nlb_apply(T *__restrict__ out, long ldo, 
     }
 }
 
+// ---- sphere layout: one block per (influence entry, column block) ----------
+//
+// For projectors whose spheres cover a small part of a brick (real
+// pseudopotentials, rc ~ 2 Bohr: ~500 points against ~6000 brick points) the
+// work follows the reference's gather / GEMM / scatter per atom instead:
+//
+//  1. nls_project: part[e][p][col] = sum_q Chi_e[q][p] X[col][pos_e[q]] on
+//     16x16x4 MFMA; the entry's points are staged in LDS 256 at a time
+//     (gathered, consecutive threads on consecutive points), the four waves
+//     split each chunk and their tiles are summed in a fixed order.
+//  2. nls_apply, one launch per colour: beta = scale dV Gamma sum over the
+//     atom's images of part (fixed order), then out[col][pos_e[q]] +=
+//     sum_p Chi_e[q][p] beta[p][col] as the transposed tile (cols x points)
+//     so that every output register row is 16 consecutive points of one
+//     column.  Entries of one colour share no grid point, so the
+//     read-add-write needs no atomics and the result is deterministic.
+constexpr int kQC = 256;           // points staged per chunk (multiple of 64)
+constexpr int kQLD = kQC + 4;      // LDS stride of a staged column
+
+template <typename T>
+__global__ __launch_bounds__(256) void nls_project(const T *__restrict__ X, long ldx, int ncol,
+                                                   const int *__restrict__ ent_info,
+                                                   const long *__restrict__ ent_chi,
+                                                   const int *__restrict__ gpos,
+                                                   const T *__restrict__ chir, int max_np,
+                                                   T *__restrict__ part)
+{
+    using M = Mfma<T>;
+    constexpr int NC = M::NCP, NCT = NC / 16, MRT = 4;
+    __shared__ __align__(16) T xs[NC * kQLD];
+    __shared__ typename M::acc red[MRT * NCT][64];
+    const int lid = xcd_remap(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int e = lid / gridDim.x;
+    const int col0 = (lid % gridDim.x) * NC;
+    const int qpad = ent_info[6 * e], poff = ent_info[6 * e + 1];
+    const int np = ent_info[6 * e + 2], npad = ent_info[6 * e + 3];
+    const int nrt = npad / 16;
+    const T *ch = chir + ent_chi[e];
+    const int *gp = gpos + poff;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    typename M::acc acc[MRT][NCT];
+#pragma unroll
+    for (int rt = 0; rt < MRT; rt++)
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++) acc[rt][ct] = typename M::acc{0, 0, 0, 0};
+    for (int qc = 0; qc < qpad; qc += kQC) {
+        const int qn = min(kQC, qpad - qc);
+        for (int t = threadIdx.x; t < qn * NC; t += 256) {
+            const int c = t / qn, q = t - c * qn;
+            const int g = gp[qc + q];
+            xs[c * kQLD + q] = (g >= 0 && col0 + c < ncol) ? X[(long)(col0 + c) * ldx + g] : T(0);
+        }
+        __syncthreads();
+        const int qw = qn / 4;
+        for (int k = wave * qw; k < (wave + 1) * qw; k += 4) {
+            const int q = k + lk;
+            T bv[NCT];
+#pragma unroll
+            for (int ct = 0; ct < NCT; ct++) bv[ct] = xs[(ct * 16 + li) * kQLD + q];
+            const T *arow = ch + (long)(qc + q) * npad + li;
+#pragma unroll
+            for (int rt = 0; rt < MRT; rt++) {
+                if (rt < nrt) {
+                    const T a = arow[rt * 16];
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ct++) acc[rt][ct] = M::run(a, bv[ct], acc[rt][ct]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // fixed-order cross-wave sum: wave 3 -> 2 -> 1 -> 0
+    for (int w = kWaves - 1; w >= 1; w--) {
+        if (wave == w)
+#pragma unroll
+            for (int rt = 0; rt < MRT; rt++)
+#pragma unroll
+                for (int ct = 0; ct < NCT; ct++) red[rt * NCT + ct][lane] = acc[rt][ct];
+        __syncthreads();
+        if (wave == w - 1)
+#pragma unroll
+            for (int rt = 0; rt < MRT; rt++)
+#pragma unroll
+                for (int ct = 0; ct < NCT; ct++) acc[rt][ct] += red[rt * NCT + ct][lane];
+        __syncthreads();
+    }
+    if (wave == 0) {
+#pragma unroll
+        for (int rt = 0; rt < MRT; rt++)
+#pragma unroll
+            for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int p = rt * 16 + M::drow(lk, r);
+                    const int col = col0 + ct * 16 + li;
+                    if (p < np && col < ncol) part[((long)e * max_np + p) * ncol + col] = acc[rt][ct][r];
+                }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void nls_apply(T *__restrict__ out, long ldo, int ncol, int e0,
+                                                 const int *__restrict__ ent_info,
+                                                 const long *__restrict__ ent_chi,
+                                                 const int *__restrict__ gpos,
+                                                 const T *__restrict__ chit,
+                                                 const int *__restrict__ atom_img_ptr,
+                                                 const int *__restrict__ atom_img,
+                                                 const int *__restrict__ atom_row,
+                                                 const double *__restrict__ gamma,
+                                                 double scale_dv, const T *__restrict__ part,
+                                                 int max_np)
+{
+    using M = Mfma<T>;
+    constexpr int NC = M::NCA, NCT = NC / 16, LDB = NC * 3 / 2;
+    __shared__ T beta[64 * LDB];
+    const int lid = xcd_remap(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int e = e0 + lid / gridDim.x;
+    const int col0 = (lid % gridDim.x) * NC;
+    const int qpad = ent_info[6 * e], poff = ent_info[6 * e + 1];
+    const int np = ent_info[6 * e + 2], npad = ent_info[6 * e + 3], atom = ent_info[6 * e + 4];
+    const int i0 = atom_img_ptr[atom], i1 = atom_img_ptr[atom + 1], row = atom_row[atom];
+    for (int t = threadIdx.x; t < npad * NC; t += 256) {
+        const int p = t / NC, c = t - p * NC;
+        T v = T(0);
+        if (p < np && col0 + c < ncol) {
+            for (int i = i0; i < i1; i++) v += part[((long)atom_img[i] * max_np + p) * ncol + col0 + c];
+            v *= (T)(scale_dv * gamma[row + p]);
+        }
+        beta[p * LDB + c] = v;
+    }
+    __syncthreads();
+    const T *ct_ = chit + ent_chi[e];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    for (int qt = wave; qt < qpad / 16; qt += kWaves) {
+        const int q0 = qt * 16;
+        typename M::acc acc[NCT];
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++) acc[ct] = typename M::acc{0, 0, 0, 0};
+        for (int j = 0; j < npad; j += 4) {
+            const T bq = ct_[(long)(j + lk) * qpad + q0 + li];  // B (p, point)
+#pragma unroll
+            for (int ct = 0; ct < NCT; ct++)                    // A (col, p)
+                acc[ct] = M::run(beta[(j + lk) * LDB + ct * 16 + li], bq, acc[ct]);
+        }
+        const int g = gpos[poff + q0 + li];
+        if (g < 0) continue;
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int col = col0 + ct * 16 + M::drow(lk, r);
+                if (col < ncol) out[(long)col * ldo + g] += acc[ct][r];
+            }
+        }
+    }
+}
+
+template <typename T>
+int launch_vnl_sphere(sgpu_context *ctx, int ncol, const T *x, long ldx, T *out, long ldo,
+                      double scale, const T *chir, const T *chit)
+{
+    const sgpu_nonlocal &nl = ctx->nl;
+    constexpr int NCP = Mfma<T>::NCP, NC = Mfma<T>::NCA;
+    const int max_np = 64;
+    void *buf = nullptr;
+    SGPU_CHECK(sgpu_scratch(ctx, (size_t)nl.n_ent * max_np * ncol * sizeof(T) + 256, &buf));
+    T *part = (T *)buf;
+    sgpu_tmark t;
+    SGPU_CHECK(sgpu_tbegin(ctx, &t));
+    nls_project<T><<<dim3((ncol + NCP - 1) / NCP, nl.n_ent), 256, 0, ctx->stream>>>(
+        x, ldx, ncol, nl.ent_info, nl.ent_chi, nl.gpos, chir, max_np, part);
+    SGPU_HIP(hipGetLastError());
+    SGPU_CHECK(sgpu_tend(ctx, &t, "nl_project"));
+    SGPU_CHECK(sgpu_tbegin(ctx, &t));
+    for (int c = 0; c < nl.n_colors; c++) {
+        const int e0 = nl.color_start[c], ne = nl.color_start[c + 1] - e0;
+        if (ne <= 0) continue;
+        nls_apply<T><<<dim3((ncol + NC - 1) / NC, ne), 256, 0, ctx->stream>>>(
+            out, ldo, ncol, e0, nl.ent_info, nl.ent_chi, nl.gpos, chit, nl.atom_img_ptr,
+            nl.atom_img, nl.atom_row, nl.gamma, scale * nl.dV, part, max_np);
+        SGPU_HIP(hipGetLastError());
+    }
+    SGPU_CHECK(sgpu_tend(ctx, &t, "nl_apply"));
+    return SGPU_OK;
+}
+
 template <typename T>
 int launch_vnl_typed(sgpu_context *ctx, int ncol, const T *x, long ldx, T *out, long ldo,
                      double scale, const T *phi, const T *phic)
@@ -289,12 +479,121 @@ int upload(T **dst, const T *src, size_t n, hipStream_t s)
     return SGPU_OK;
 }
 
+// Sphere layout (see nls_project / nls_apply): entries greedily coloured so
+// that no two entries of a colour share a grid point, each entry's point
+// list padded to a multiple of 64 (-1 = padding) and its Chi stored twice,
+// [q][npad] for the projection and [p][qpad] for the apply.
+int build_sphere_layout(sgpu_context *ctx, int n_influence, const int *ndc, const int *nproj,
+                        const long *pos_off, const int *grid_pos, const long *chi_off,
+                        const double *chi, const int *alpha_off, int n_alpha,
+                        const double *gamma, double dV)
+{
+    // atoms = distinct alpha offsets (images of one atom share them)
+    std::vector<int> offs(alpha_off, alpha_off + n_influence);
+    std::sort(offs.begin(), offs.end());
+    offs.erase(std::unique(offs.begin(), offs.end()), offs.end());
+    const int n_atoms = (int)offs.size();
+    auto atom_of = [&](int a) {
+        return (int)(std::lower_bound(offs.begin(), offs.end(), alpha_off[a]) - offs.begin());
+    };
+    // greedy colouring over shared grid points
+    std::vector<uint64_t> used((size_t)ctx->nd, 0);
+    std::vector<int> color(n_influence);
+    int n_colors = 0;
+    for (int a = 0; a < n_influence; a++) {
+        uint64_t m = 0;
+        for (int i = 0; i < ndc[a]; i++) m |= used[grid_pos[pos_off[a] + i]];
+        if (m == ~0ull) {
+            sgpu_set_error("sgpu_set_nonlocal: more than 64 overlapping projector spheres");
+            return SGPU_ERR_ARG;
+        }
+        int c = 0;
+        while (m & (1ull << c)) c++;
+        color[a] = c;
+        n_colors = std::max(n_colors, c + 1);
+        for (int i = 0; i < ndc[a]; i++) used[grid_pos[pos_off[a] + i]] |= 1ull << c;
+    }
+    std::vector<int> order(n_influence);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return color[x] < color[y]; });
+    std::vector<int> color_start(n_colors + 1, 0);
+    for (int a = 0; a < n_influence; a++) color_start[color[a] + 1]++;
+    for (int c = 0; c < n_colors; c++) color_start[c + 1] += color_start[c];
+    // per entry layout
+    std::vector<int> info(6 * (size_t)n_influence), gp;
+    std::vector<long> eoff(n_influence);
+    std::vector<double> chir, chit;
+    std::vector<std::vector<int>> imgs(n_atoms);
+    long coff = 0;
+    int max_qpad = 0;
+    for (int e = 0; e < n_influence; e++) {
+        const int a = order[e];
+        const int n = ndc[a], np = nproj[a];
+        const int qpad = std::max(64, (n + 63) / 64 * 64), npad = std::max(16, (np + 15) / 16 * 16);
+        const int atom = atom_of(a);
+        info[6 * e] = qpad;
+        info[6 * e + 1] = (int)gp.size();
+        info[6 * e + 2] = np;
+        info[6 * e + 3] = npad;
+        info[6 * e + 4] = atom;
+        info[6 * e + 5] = 0;
+        eoff[e] = coff;
+        imgs[atom].push_back(e);
+        max_qpad = std::max(max_qpad, qpad);
+        for (int q = 0; q < qpad; q++) gp.push_back(q < n ? grid_pos[pos_off[a] + q] : -1);
+        chir.resize(coff + (size_t)qpad * npad, 0.0);
+        chit.resize(coff + (size_t)qpad * npad, 0.0);
+        for (int p = 0; p < np; p++)
+            for (int q = 0; q < n; q++) {
+                const double v = chi[chi_off[a] + (long)p * n + q];
+                chir[coff + (long)q * npad + p] = v;
+                chit[coff + (long)p * qpad + q] = v;
+            }
+        coff += (long)qpad * npad;
+    }
+    std::vector<int> img_ptr(n_atoms + 1, 0), img, arow(n_atoms);
+    for (int t = 0; t < n_atoms; t++) {
+        img_ptr[t + 1] = img_ptr[t] + (int)imgs[t].size();
+        img.insert(img.end(), imgs[t].begin(), imgs[t].end());
+        arow[t] = offs[t];
+    }
+    sgpu_nonlocal &nl = ctx->nl;
+    hipStream_t s = ctx->stream;
+    nl.mode = 1;
+    nl.n_infl = n_influence;
+    nl.n_alpha = n_alpha;
+    nl.n_ent = n_influence;
+    nl.n_atoms = n_atoms;
+    nl.n_colors = n_colors;
+    nl.max_qpad = max_qpad;
+    nl.color_start = color_start;
+    nl.dV = dV;
+    SGPU_CHECK(upload(&nl.ent_info, info.data(), info.size(), s));
+    SGPU_CHECK(upload(&nl.ent_chi, eoff.data(), eoff.size(), s));
+    SGPU_CHECK(upload(&nl.gpos, gp.data(), gp.size(), s));
+    SGPU_CHECK(upload(&nl.chir64, chir.data(), chir.size(), s));
+    SGPU_CHECK(upload(&nl.chit64, chit.data(), chit.size(), s));
+    SGPU_HIP(hipMalloc((void **)&nl.chir32, std::max<size_t>(chir.size(), 1) * sizeof(float)));
+    SGPU_HIP(hipMalloc((void **)&nl.chit32, std::max<size_t>(chit.size(), 1) * sizeof(float)));
+    SGPU_CHECK(sgpu_convert(ctx, nl.chir64, SGPU_F64, nl.chir32, SGPU_F32, (long)chir.size()));
+    SGPU_CHECK(sgpu_convert(ctx, nl.chit64, SGPU_F64, nl.chit32, SGPU_F32, (long)chit.size()));
+    SGPU_CHECK(upload(&nl.atom_img_ptr, img_ptr.data(), img_ptr.size(), s));
+    SGPU_CHECK(upload(&nl.atom_img, img.data(), img.size(), s));
+    SGPU_CHECK(upload(&nl.atom_row, arow.data(), arow.size(), s));
+    SGPU_CHECK(upload(&nl.gamma, gamma, (size_t)n_alpha, s));
+    SGPU_HIP(hipStreamSynchronize(s));
+    return SGPU_OK;
+}
+
 }  // namespace
 
 void sgpu_free_nonlocal(sgpu_nonlocal &nl)
 {
-    void *ptrs[] = {nl.brick_org, nl.brick_np, nl.brick_row, nl.row_alpha, nl.phi64, nl.phi32,
-                    nl.phic64,    nl.phic32,   nl.arow_ptr,  nl.arow_src, nl.gamma};
+    void *ptrs[] = {nl.brick_org, nl.brick_np, nl.brick_row,  nl.row_alpha, nl.phi64,
+                    nl.phi32,     nl.phic64,   nl.phic32,     nl.arow_ptr,  nl.arow_src,
+                    nl.gamma,     nl.ent_info, nl.ent_chi,    nl.gpos,      nl.chir64,
+                    nl.chir32,    nl.chit64,   nl.chit32,     nl.atom_img_ptr, nl.atom_img,
+                    nl.atom_row};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     nl = sgpu_nonlocal();
@@ -342,6 +641,27 @@ extern "C" int sgpu_set_nonlocal(sgpu_context *ctx, int n_influence, const int *
         for (int b : bl) pairs.push_back({b, a});
     }
     std::sort(pairs.begin(), pairs.end());
+    {
+        // layout choice: the sphere layout when the spheres fill less than a
+        // quarter of the bricks they touch (SPARC_AMD_NL_MODE=brick|sphere
+        // overrides); it needs at most 64 projectors per entry
+        long pts = 0;
+        int np_max = 0;
+        for (int a = 0; a < n_influence; a++) {
+            pts += ndc[a];
+            np_max = std::max(np_max, nproj[a]);
+        }
+        const double fill = pairs.empty() ? 1.0 : (double)pts / ((double)pairs.size() * kBP);
+        int mode = fill < 0.25 ? 1 : 0;
+        if (const char *env = getenv("SPARC_AMD_NL_MODE")) {
+            if (!strcmp(env, "brick")) mode = 0;
+            if (!strcmp(env, "sphere")) mode = 1;
+        }
+        if (np_max > 64) mode = 0;
+        if (mode == 1)
+            return build_sphere_layout(ctx, n_influence, ndc, nproj, pos_off, grid_pos, chi_off, chi,
+                                       alpha_off, n_alpha, gamma, dV);
+    }
     // bricks, their column layout and row offsets
     std::vector<int> bid, borg, bnp;
     std::vector<long> brow;
@@ -455,6 +775,14 @@ extern "C" int sgpu_set_nonlocal(sgpu_context *ctx, int n_influence, const int *
 int sgpu_launch_vnl(sgpu_context *ctx, int dtype, int ncol, const void *x, long ldx, void *out,
                     long ldo, double scale)
 {
+    if (ctx->nl.mode == 1) {
+        if (ctx->nl.n_ent == 0 || ncol <= 0) return SGPU_OK;
+        if (dtype == SGPU_F64)
+            return launch_vnl_sphere<double>(ctx, ncol, (const double *)x, ldx, (double *)out, ldo,
+                                             scale, ctx->nl.chir64, ctx->nl.chit64);
+        return launch_vnl_sphere<float>(ctx, ncol, (const float *)x, ldx, (float *)out, ldo, scale,
+                                        ctx->nl.chir32, ctx->nl.chit32);
+    }
     if (dtype == SGPU_F64)
         return launch_vnl_typed<double>(ctx, ncol, (const double *)x, ldx, (double *)out, ldo,
                                         scale, ctx->nl.phi64, ctx->nl.phic64);

@@ -77,6 +77,22 @@ struct sgpu_nonlocal {
     int *arow_src = nullptr;
     double *gamma = nullptr;    // Gamma per alpha row
     double dV = 0.0;
+    // sphere layout (small projector spheres): per influence entry its point
+    // list and Chi, entries ordered by colour (no two entries of a colour
+    // share a grid point), images of one atom found through atom_img
+    int mode = 0;               // 0 brick layout, 1 sphere layout
+    int n_ent = 0, n_atoms = 0, n_colors = 0, max_qpad = 0;
+    std::vector<int> color_start;   // host: first entry of each colour (+ end)
+    int *ent_info = nullptr;    // per entry: qpad, pos offset, np, npad, atom, chi offset (6 ints)
+    long *ent_chi = nullptr;    // per entry: element offset of its Chi blocks
+    int *gpos = nullptr;        // grid index per padded point (-1: padding)
+    double *chir64 = nullptr;   // Chi [q][npad] per entry
+    float *chir32 = nullptr;
+    double *chit64 = nullptr;   // Chi [p][qpad] per entry
+    float *chit32 = nullptr;
+    int *atom_img_ptr = nullptr;  // atom a: entries atom_img[ptr[a]..ptr[a+1])
+    int *atom_img = nullptr;
+    int *atom_row = nullptr;      // alpha row of the atom's first projector
 };
 
 struct sgpu_timer_acc {

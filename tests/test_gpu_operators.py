@@ -122,3 +122,56 @@ def test_uniform_fill_matches_host_hash(gpu):
     h = uniform_block(g.nd, 5, 1234, -0.5, 0.5)
     assert np.array_equal(d, h)
     assert np.array_equal(d32, h.astype(np.float32).astype(np.float64))
+
+
+@pytest.fixture
+def nl_mode(monkeypatch):
+    """Force the nonlocal layout (read when the projectors are set)."""
+    def set_mode(mode):
+        monkeypatch.setenv("SPARC_AMD_NL_MODE", mode)
+    return set_mode
+
+
+@pytest.mark.parametrize("mode", ["brick", "sphere"])
+@pytest.mark.parametrize("images", [False, True])
+def test_vnl_layouts_fp64(gpu, oracle_mod, nl_mode, mode, images):
+    """Both projector layouts against the oracle, including atoms whose
+    periodic images share alpha rows and overlapping spheres."""
+    nl_mode(mode)
+    g, veff, nl, rng = small_case(n=(16, 15, 14), n_sites=6, rc=2.6, seed=13, lmax=2)
+    if images:
+        nl = split_images(nl, 1)
+    o = oracle_mod.Oracle(g, veff, nl)
+    x = rng.standard_normal((g.nd, 37))
+    h0 = rng.standard_normal((g.nd, 37))
+    with gpu.Context(g) as ctx:
+        ctx.set_nonlocal(nl)
+        y = ctx.Vnl_vec_mult(x, h0)
+    yo = o.vnl_vec_mult(x, h0)
+    assert rel_err(y - h0, yo - h0) < 1e-12
+
+
+@pytest.mark.parametrize("mode", ["brick", "sphere"])
+def test_vnl_psp_projectors(gpu, oracle_mod, nl_mode, mode):
+    """Real psp8 projectors (SiC cell: 18 / 8 projectors per atom, images
+    across the cell faces) in both layouts, FP64 and FP32 blocks."""
+    from sparc_amd.systems import sic8
+    nl_mode(mode)
+    st, _ = sic8()
+    g = st.grid
+    rng = np.random.default_rng(17)
+    veff = np.zeros(g.nd)
+    o = oracle_mod.Oracle(g, veff, st.nloc)
+    x = rng.uniform(-0.5, 0.5, (g.nd, 40))
+    yo = o.vnl_vec_mult(x)
+    with gpu.Context(g) as ctx:
+        ctx.set_veff(veff)
+        ctx.set_nonlocal(st.nloc)
+        y = ctx.Vnl_vec_mult(x)
+        bx = ctx.block(40, gpu.F32).upload(x)
+        by = ctx.block(40, gpu.F32)
+        ctx.hamiltonian_dev(0.0, bx, by)
+        y32 = by.download()
+    assert rel_err(y, yo) < 1e-12
+    x32 = x.astype(np.float32).astype(np.float64)
+    assert rel_err(y32, o.hamiltonian(x32)) < 2e-6
