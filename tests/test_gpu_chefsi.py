@@ -144,3 +144,26 @@ def test_chefsi_converges_to_dense_spectrum(gpu, oracle_mod, dtype, tol):
             a0, a = lam[0], lam[-1] + 0.1
     nconv = ns - 4
     assert np.max(np.abs(lam[:nconv] - ev[:nconv])) < tol
+
+
+def test_generalized_eigensolve_filtered_scale(gpu):
+    """n = 480 with the spread of scales a Chebyshev-filtered block gives
+    (Mp diagonal over eight decades, nearly diagonal Hp): eigenvalues against
+    LAPACK dsygvd, Mp-orthonormal eigenvectors; and the not-positive-definite
+    error path."""
+    rng = np.random.default_rng(31)
+    n = 480
+    d = 10.0 ** rng.uniform(-4, 4, n)
+    lam0 = np.sort(rng.uniform(-0.3, 0.5, n))
+    E = 1e-3 * rng.standard_normal((n, n))
+    Mp = np.sqrt(np.outer(d, d)) * (np.eye(n) + (E + E.T) / 2)
+    F = 1e-3 * rng.standard_normal((n, n))
+    Hp = np.sqrt(np.outer(d, d)) * (np.diag(lam0) + (F + F.T) / 2)
+    from sparc_amd.workloads import Grid
+    with gpu.Context(Grid(8, 8, 8, 0.5, 0.5, 0.5)) as ctx:
+        lam, Q = ctx.DP_Solve_Generalized_EigenProblem(Hp, Mp)
+        with pytest.raises(gpu.SparcGPUError):
+            ctx.DP_Solve_Generalized_EigenProblem(Hp, -Mp)
+    lam_ref = sla.eigh(Hp, Mp, eigvals_only=True, driver="gvd")
+    assert np.max(np.abs(lam - lam_ref)) < 1e-10
+    assert np.max(np.abs(Q.T @ Mp @ Q - np.eye(n))) < 1e-9
