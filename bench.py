@@ -171,6 +171,73 @@ def scf_bench(name):
     return out
 
 
+def poisson_bench(train_steps=400):
+    """AAR Poisson solve on the 128^3 / 216-charge case with the Jacobi
+    preconditioner and with the learned U-Net preconditioner, the network
+    trained here on the GPU on pairs harvested from the 64-atom Si SCF's
+    Poisson solves (52^3 grid; the 8-atom cell's 26^3 grid is not a multiple
+    of 4, which the U-Net's two halvings need)."""
+    from sparc_amd import gpu
+    from sparc_amd.systems import scf_system
+    from sparc_amd.workloads import poisson_workload
+    out = {}
+    st, p = scf_system("si64")
+    t0 = time.perf_counter()
+    with gpu.Context(st.grid) as ctx:
+        ctx.set_nonlocal(st.nloc)
+        rhos = [st.rho_at]
+        for k in (1, 2, 4):
+            r = ctx.scf_run(st.b, st.rho_at, dict(p, max_scf=k, dtype=gpu.F32), rho_core=st.rho_core)
+            rhos.append(r["rho"])
+        net = ctx.unet(seed=1)
+        th = time.perf_counter()
+        for rho in rhos:
+            net.harvest(4.0 * np.pi * (st.b + rho), np.zeros(st.grid.nd), tol=1e-8, max_iter=3000,
+                        stride=2, max_pairs=120)
+        th = time.perf_counter() - th
+        tt = time.perf_counter()
+        hist = net.train(train_steps, batch=4, lr=2e-3, seed=5)
+        tt = time.perf_counter() - tt
+        w = net.get_weights()
+        out["training"] = {"pairs": net.npairs, "steps": train_steps, "batch": 4,
+                           "harvest_s": round(th, 2), "train_s": round(tt, 2),
+                           "loss_first": float(hist[:5].mean()), "loss_last": float(hist[-20:].mean()),
+                           "source": "Jacobi-AAR Poisson solves (tol 1e-8) of the si64 SCF's "
+                                     "densities (guess and after 1, 2, 4 SCF iterations), 52^3"}
+        net.close()
+    print(f"[bench] unet: {out['training']['pairs']} pairs, harvest {th:.1f}s, train {tt:.1f}s, "
+          f"loss {out['training']['loss_first']:.3e} -> {out['training']['loss_last']:.3e}",
+          flush=True)
+    g, rhs = poisson_workload()
+    out["workload"] = ("AAR Poisson, 128^3 periodic h=0.35, 216 Gaussian charges q=4 widths "
+                       "0.8-1.5 neutralised, rel tol 1e-8, AAR 0.6/0.6/7/6, x0 = 0")
+    with gpu.Context(g) as ctx:
+        net = ctx.unet(seed=1)
+        net.set_weights(w)
+        x0 = np.zeros(g.nd)
+        ctx.AAR(x0, rhs, tol=1e-2, max_iter=50)          # warm-up (allocations, code load)
+        ctx.AAR_unet(net, x0, rhs, tol=1e-2, max_iter=50)
+        ctx.sync()
+        t = time.perf_counter()
+        xj, itj, rrj, okj = ctx.AAR(x0, rhs, tol=1e-8, max_iter=10000)
+        tj = time.perf_counter() - t
+        t = time.perf_counter()
+        xu, itu, rru, stu = ctx.AAR_unet(net, x0, rhs, tol=1e-8, max_iter=10000)
+        tu = time.perf_counter() - t
+        net.close()
+    xj -= xj.mean()
+    xu -= xu.mean()
+    out["jacobi"] = {"iterations": itj, "ms": round(tj * 1e3, 2), "relres": rrj, "converged": okj}
+    out["unet"] = {"iterations": itu, "ms": round(tu * 1e3, 2), "relres": rru,
+                   "converged": stu == 0}
+    out["solution_rel_diff"] = float(np.linalg.norm(xu - xj) / np.linalg.norm(xj))
+    out["iteration_ratio"] = round(itj / max(itu, 1), 3)
+    out["time_ratio"] = round(tj / tu, 3)
+    print(f"[bench] poisson 128^3: Jacobi {itj} it {tj * 1e3:.1f} ms, U-Net {itu} it "
+          f"{tu * 1e3:.1f} ms", flush=True)
+    return out
+
+
 def cpu_baseline_scf(name="si8"):
     """The FP64 oracle SCF of the 8-atom Si cell on the host cores (whole run)."""
     from oracle import oracle as O
@@ -203,6 +270,8 @@ def main(argv=None):
     ap.add_argument("--degree", type=int, default=24)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-poisson", action="store_true",
+                    help="skip the Jacobi vs learned-preconditioner Poisson comparison")
     ap.add_argument("--scf", default="si64,sial216",
                     help="comma-separated SCF workloads to time (sic8, si8, si64, sial216) or "
                          "'none'")
@@ -287,6 +356,8 @@ def main(argv=None):
     ctx.close()
     if args.scf and args.scf != "none":
         res["scf"] = {name: scf_bench(name) for name in args.scf.split(",")}
+    if not args.no_poisson:
+        res["poisson_learned_precond"] = poisson_bench()
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", flush=True)
         res["cpu_baseline"] = cpu_baseline(wl)

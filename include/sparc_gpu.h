@@ -236,6 +236,39 @@ int sgpu_scf_run(sgpu_context *ctx, const sgpu_scf_params *par, const double *b,
                  double *phi_out,
                  double *iter_ms, double *iter_err, double *iter_etot, sgpu_scf_result *res);
 
+/* ---- learned AAR preconditioner --------------------------------------- */
+
+/* A linear 3-level 3-D convolutional U-Net (16 channels, BF16 activations,
+ * FP32 accumulation and master weights) bound to a context's periodic grid
+ * (every dimension a multiple of 4).  Applied as f = (r + U(r)) / |D0| with
+ * D0 = D2x[0] + D2y[0] + D2z[0]: the reference's Jacobi_preconditioner
+ * (src/electrostatics.c:1682-1699) is the U = 0 case, which is also the
+ * network's initial state.  No counterpart in the reference beyond that. */
+typedef struct sgpu_unet sgpu_unet;
+int  sgpu_unet_create(sgpu_context *ctx, int channels, unsigned long long seed, sgpu_unet **net);
+void sgpu_unet_destroy(sgpu_unet *net);
+int  sgpu_unet_nparams(const sgpu_unet *net);
+int  sgpu_unet_get_weights(const sgpu_unet *net, float *w);
+int  sgpu_unet_set_weights(sgpu_unet *net, const float *w);
+/* A precond_fun for sgpu_AAR (user = the sgpu_unet*), the signature of the
+ * reference's precond_fun (src/linearSolver.c:37-43): FP64 device vectors. */
+void sgpu_unet_precond(void *user, int N, double c, const double *r_dev, double *f_dev);
+/* f = P(r) on host vectors of length Nd. */
+int  sgpu_unet_apply(sgpu_unet *net, const double *r, double *f);
+/* Loss mean_i |P(r_i) - e_i|^2 over npair host pairs (Nd each, stacked) and
+ * its gradient with respect to the weights (sgpu_unet_nparams floats). */
+int  sgpu_unet_loss_grad(sgpu_unet *net, const double *r, const double *e, int npair,
+                         double *loss, float *grad);
+/* Training pairs from a Jacobi-AAR solve of the context's Poisson problem
+ * (rhs b, start x0, host vectors): every stride-th iterate k gives
+ * (r_k / |e_k|, (e_k - mean e_k) / |e_k|) with e_k = x* - x_k. */
+int  sgpu_unet_harvest(sgpu_unet *net, const double *b, const double *x0, double tol,
+                       int max_iter, int stride, int max_pairs, int *added);
+int  sgpu_unet_npairs(const sgpu_unet *net);
+/* Adam (lr, 0.9, 0.999) over the stored pairs, batch pairs per step. */
+int  sgpu_unet_train(sgpu_unet *net, int steps, int batch, double lr, unsigned long long seed,
+                     double *loss_hist);
+
 /* ---- timing ---------------------------------------------------------- */
 
 /* Average device time (ms) of the last timed launches of the named kernel

@@ -216,7 +216,8 @@ int sgpu_anderson_update_dev(sgpu_context *ctx, double *x, const double *xk, con
 // AAR on device vectors x (in/out) and b, both of length N = Nd.
 int sgpu_aar_device(sgpu_context *ctx, sgpu_res_fn res, sgpu_precond_fn precond, void *user,
                     double c, int N, double *x, const double *b, double omega, double beta, int m,
-                    int p, double tol, int max_iter, int *iters, double *relres)
+                    int p, double tol, int max_iter, int *iters, double *relres,
+                    const sgpu_aar_recorder *rec)
 {
     if (!ctx || !x || !b || N != ctx->nd || m < 1 || m > kAarMaxM || p < 1 || max_iter < 0) {
         sgpu_set_error("AAR: bad arguments (N %d, Nd %ld, m %d, p %d)", N, ctx ? ctx->nd : 0L, m, p);
@@ -254,6 +255,7 @@ int sgpu_aar_device(sgpu_context *ctx, sgpu_res_fn res, sgpu_precond_fn precond,
                                            ld, 1.0, 1.0, true));
             SGPU_CHECK(sgpu_tend(ctx, &t, "aar_residual"));
         }
+        if (rec && rec->fn) rec->fn(rec->user, x, r);
         return SGPU_OK;
     };
     SGPU_HIP(hipMemsetAsync(Xh, 0, 2 * (size_t)m * ld * sizeof(double), ctx->stream));

@@ -188,9 +188,16 @@ int sgpu_anderson_update_dev(sgpu_context *ctx, double *x, const double *xk, con
                              const double *Xh, const double *Fh, long ldh, int m, const double *G,
                              double beta, long n);
 // AAR on device vectors (x in/out, b), returns SGPU_OK / SGPU_NOT_CONVERGED / error.
+// Optional observer of the AAR iterates: fn(user, x, r) after every residual
+// evaluation (r = b + (Lap + c) x), on the context's stream.
+struct sgpu_aar_recorder {
+    void *user = nullptr;
+    void (*fn)(void *user, const double *x_dev, const double *r_dev) = nullptr;
+};
 int sgpu_aar_device(sgpu_context *ctx, sgpu_res_fn res, sgpu_precond_fn precond, void *user,
                     double c, int N, double *x, const double *b, double omega, double beta, int m,
-                    int p, double tol, int max_iter, int *iters, double *relres);
+                    int p, double tol, int max_iter, int *iters, double *relres,
+                    const sgpu_aar_recorder *rec = nullptr);
 // ---- blas.hip ----
 int sgpu_dot_many(sgpu_context *ctx, int n, int npairs, const double *const *a,
                   const double *const *b, double *host_out);

@@ -115,6 +115,17 @@ _SIGS = {
     "sgpu_chefsi_dev": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_double, C.c_double, C.c_double, _dp]),
     "sgpu_scf_run": (C.c_int, [_vp, C.POINTER(SCFParams), _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                _dp, C.POINTER(SCFResult)]),
+    "sgpu_unet_create": (C.c_int, [_vp, C.c_int, C.c_ulonglong, C.POINTER(_vp)]),
+    "sgpu_unet_destroy": (None, [_vp]),
+    "sgpu_unet_nparams": (C.c_int, [_vp]),
+    "sgpu_unet_get_weights": (C.c_int, [_vp, C.POINTER(C.c_float)]),
+    "sgpu_unet_set_weights": (C.c_int, [_vp, C.POINTER(C.c_float)]),
+    "sgpu_unet_precond": (None, [_vp, C.c_int, C.c_double, _dp, _dp]),
+    "sgpu_unet_apply": (C.c_int, [_vp, _dp, _dp]),
+    "sgpu_unet_loss_grad": (C.c_int, [_vp, _dp, _dp, C.c_int, _dp, C.POINTER(C.c_float)]),
+    "sgpu_unet_harvest": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_int, C.c_int, C.c_int, _ip]),
+    "sgpu_unet_npairs": (C.c_int, [_vp]),
+    "sgpu_unet_train": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, C.c_ulonglong, _dp]),
     "sgpu_timer_get": (C.c_int, [_vp, C.c_char_p, _dp, _ip]),
     "sgpu_timer_reset": (C.c_int, [_vp]),
     "sgpu_timer_enable": (C.c_int, [_vp, C.c_int]),
@@ -404,6 +415,22 @@ class Context:
                    iter_etot=iet[:k], status=st)
         return out
 
+    # -- learned preconditioner
+    def unet(self, seed: int = 1) -> "UNet":
+        return UNet(self, seed)
+
+    def AAR_unet(self, net: "UNet", x0, b, tol: float = 1e-8, max_iter: int = 3000):
+        """AAR with the U-Net as precond_fun (the reference's Poisson AAR
+        parameters 0.6 / 0.6 / 7 / 6).  Returns (x, iterations, relres)."""
+        x = np.array(x0, np.float64, copy=True).ravel()
+        bb = _f64(b).ravel()
+        it, rr = C.c_int(), C.c_double()
+        pre = C.cast(self.lib.sgpu_unet_precond, C.c_void_p)
+        st = self._check(self.lib.sgpu_AAR(self.h, None, pre, net.h, 0.0, self.nd, _ptr(x), _ptr(bb),
+                                           0.6, 0.6, 7, 6, tol, max_iter, C.byref(it),
+                                           C.byref(rr)))
+        return x, it.value, rr.value, st
+
     # -- timers
     def timer_enable(self, on: bool = True):
         self._check(self.lib.sgpu_timer_enable(self.h, 1 if on else 0))
@@ -415,3 +442,71 @@ class Context:
         t, n = C.c_double(), C.c_int()
         self._check(self.lib.sgpu_timer_get(self.h, name.encode(), C.byref(t), C.byref(n)))
         return t.value, n.value
+
+
+class UNet:
+    """The learned AAR preconditioner (sgpu_unet_*) bound to one context."""
+
+    def __init__(self, ctx: Context, seed: int = 1):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        h = C.c_void_p()
+        ctx._check(self.lib.sgpu_unet_create(ctx.h, 16, seed, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sgpu_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def nparams(self) -> int:
+        return self.lib.sgpu_unet_nparams(self.h)
+
+    def get_weights(self) -> np.ndarray:
+        w = np.zeros(self.nparams, np.float32)
+        self.ctx._check(self.lib.sgpu_unet_get_weights(self.h, w.ctypes.data_as(C.POINTER(C.c_float))))
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.size == self.nparams
+        self.ctx._check(self.lib.sgpu_unet_set_weights(self.h, w.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def apply(self, r) -> np.ndarray:
+        r = _f64(r).ravel()
+        f = np.zeros_like(r)
+        self.ctx._check(self.lib.sgpu_unet_apply(self.h, _ptr(r), _ptr(f)))
+        return f
+
+    def loss_grad(self, R, E):
+        R = np.ascontiguousarray(R, np.float64)
+        E = np.ascontiguousarray(E, np.float64)
+        npair = R.shape[0]
+        loss = C.c_double()
+        g = np.zeros(self.nparams, np.float32)
+        self.ctx._check(self.lib.sgpu_unet_loss_grad(self.h, _ptr(R), _ptr(E), npair, C.byref(loss),
+                                                     g.ctypes.data_as(C.POINTER(C.c_float))))
+        return loss.value, g
+
+    def harvest(self, b, x0, tol=1e-8, max_iter=3000, stride=2, max_pairs=64) -> int:
+        added = C.c_int()
+        self.ctx._check(self.lib.sgpu_unet_harvest(self.h, _ptr(_f64(b).ravel()),
+                                                   _ptr(_f64(x0).ravel()), tol, max_iter, stride,
+                                                   max_pairs, C.byref(added)))
+        return added.value
+
+    @property
+    def npairs(self) -> int:
+        return self.lib.sgpu_unet_npairs(self.h)
+
+    def train(self, steps: int, batch: int = 4, lr: float = 1e-3, seed: int = 7) -> np.ndarray:
+        hist = np.zeros(max(steps, 1))
+        self.ctx._check(self.lib.sgpu_unet_train(self.h, steps, batch, lr, seed, _ptr(hist)))
+        return hist[:steps]

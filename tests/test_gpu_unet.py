@@ -1,0 +1,139 @@
+"""Learned AAR preconditioner on the device: forward pass and weight gradient
+against the FP64 oracle (oracle/unet.py) on the same weights (tolerances for
+BF16 activations: 2e-2 relative), training on harvested Jacobi-AAR pairs,
+and AAR with the trained network reaching the Jacobi-AAR solution in fewer
+iterations."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf16(a):
+    a = np.asarray(a, np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _d0abs(grid):
+    from oracle import oracle as orc
+    dx, dy, dz = orc.Oracle(grid).coeffs
+    return abs(dx[0] + dy[0] + dz[0])
+
+
+@pytest.fixture
+def grid16():
+    from sparc_amd.workloads import Grid
+    return Grid(16, 16, 16, 0.4, 0.4, 0.4)
+
+
+def test_unet_rejects_grid_not_multiple_of_4():
+    from sparc_amd import gpu
+    from sparc_amd.workloads import Grid
+    with gpu.Context(Grid(26, 26, 26, 0.4, 0.4, 0.4)) as ctx:
+        with pytest.raises(gpu.SparcGPUError):
+            ctx.unet()
+
+
+def test_unet_initial_is_jacobi(grid16):
+    from sparc_amd import gpu
+    rng = np.random.default_rng(3)
+    r = rng.standard_normal(grid16.nd)
+    with gpu.Context(grid16) as ctx:
+        f = ctx.unet(seed=5).apply(r)
+    assert np.max(np.abs(f - r / _d0abs(grid16))) < 1e-14 * np.max(np.abs(r))
+
+
+def test_unet_matrix_core_and_fma_layers_agree(grid16, monkeypatch):
+    """The MFMA layer kernels and the FP32-FMA layer kernels (selected with
+    SPARC_AMD_UNET_SCALAR=1 when the network is created) give the same
+    preconditioner up to FP32 summation order."""
+    from sparc_amd import gpu
+    rng = np.random.default_rng(6)
+    r = rng.standard_normal(grid16.nd)
+    R = rng.standard_normal((1, grid16.nd))
+    E = rng.standard_normal((1, grid16.nd)) * 0.05
+    out = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("SPARC_AMD_UNET_SCALAR", flag)
+        with gpu.Context(grid16) as ctx:
+            net = ctx.unet(seed=9)
+            net.set_weights((np.random.default_rng(8).standard_normal(net.nparams) * 0.15).astype(np.float32))
+            out[flag] = (net.apply(r), net.loss_grad(R, E))
+    (f1, (l1, g1)), (f0, (l0, g0)) = out["1"], out["0"]
+    assert np.linalg.norm(f1 - f0) < 1e-2 * np.linalg.norm(f1 - r / _d0abs(grid16))
+    assert abs(l1 - l0) < 1e-2 * l1
+    assert np.linalg.norm(g1 - g0) < 2e-2 * np.linalg.norm(g1)
+
+
+def test_unet_forward_and_gradient_match_oracle(grid16):
+    from oracle import unet as U
+    from sparc_amd import gpu
+    rng = np.random.default_rng(4)
+    d0 = _d0abs(grid16)
+    n = grid16.nx
+    with gpu.Context(grid16) as ctx:
+        net = ctx.unet(seed=9)
+        w = (rng.standard_normal(net.nparams) * 0.15).astype(np.float32)
+        net.set_weights(w)
+        r = rng.standard_normal(grid16.nd)
+        f = net.apply(r)
+        R = rng.standard_normal((2, grid16.nd))
+        E = rng.standard_normal((2, grid16.nd)) * 0.05
+        loss, g = net.loss_grad(R, E)
+    wq = _bf16(w)
+    fo, _ = U.forward(wq, _bf16(r).reshape(n, n, n), d0)
+    # the Jacobi part r/|D0| is exact; compare the network part U
+    u_gpu = f * d0 - r
+    u_ref = fo.ravel() * d0 - _bf16(r)
+    assert np.linalg.norm(u_gpu - u_ref) < 2e-2 * np.linalg.norm(u_ref)
+    Ro = [_bf16(x).reshape(n, n, n) for x in R]
+    Eo = [x.reshape(n, n, n) for x in E.astype(np.float32).astype(np.float64)]
+    lo, go = U.loss_and_grad(wq, Ro, Eo, d0)
+    assert abs(loss - lo) < 2e-2 * lo
+    cos = float(g @ go) / (np.linalg.norm(g) * np.linalg.norm(go))
+    assert cos > 0.999
+    assert np.linalg.norm(g - go) < 5e-2 * np.linalg.norm(go)
+    # per layer too (no layer's gradient may be lost in the others' norm)
+    off = 0
+    for _, _, ci, co in U.layer_shapes():
+        sl = slice(off, off + ci * co * 27)
+        assert np.linalg.norm(g[sl] - go[sl]) < 6e-2 * np.linalg.norm(go[sl]) + 1e-12
+        off += ci * co * 27
+
+
+def test_unet_trained_preconditioner_beats_jacobi():
+    """Train on pairs harvested from Jacobi-AAR Poisson solves on a 32^3
+    cell, then solve a different right-hand side on the same grid: the same
+    solution (tolerance 1e-8 relative residual for both) in fewer AAR
+    iterations."""
+    from sparc_amd import gpu
+    from sparc_amd.model import gaussian_density
+    from sparc_amd.workloads import Grid
+    g = Grid(32, 32, 32, 0.4, 0.4, 0.4)
+    rng = np.random.default_rng(11)
+
+    def rhs(seed):
+        rr = np.random.default_rng(seed)
+        pos = rr.uniform(0, 12.8, (12, 3))
+        q = gaussian_density(g, pos, 1.0, 0.9)
+        return 4 * np.pi * (q - q.mean())
+
+    with gpu.Context(g) as ctx:
+        net = ctx.unet(seed=2)
+        for s in (21, 22, 23):
+            net.harvest(rhs(s), np.zeros(g.nd), tol=1e-8, max_iter=3000, stride=3, max_pairs=60)
+        assert net.npairs > 50
+        hist = net.train(steps=300, batch=4, lr=2e-3, seed=3)
+        assert hist[-20:].mean() < 0.5 * hist[:5].mean()
+        b = rhs(99)
+        x0 = rng.standard_normal(g.nd) * 0.0
+        xj, itj, rrj, okj = ctx.AAR(x0, b, tol=1e-8, max_iter=3000)
+        xu, itu, rru, st = ctx.AAR_unet(net, x0, b, tol=1e-8, max_iter=3000)
+    print(f"\nAAR iterations: Jacobi {itj}, U-Net {itu}; final loss {hist[-1]:.3e}")
+    assert okj and st == 0 and rru <= 1e-8
+    xj -= xj.mean()
+    xu -= xu.mean()
+    assert np.linalg.norm(xu - xj) < 1e-6 * np.linalg.norm(xj)
+    assert itu < itj
