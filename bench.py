@@ -272,7 +272,7 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-poisson", action="store_true",
                     help="skip the Jacobi vs learned-preconditioner Poisson comparison")
-    ap.add_argument("--scf", default="si64,sial216",
+    ap.add_argument("--scf", default="si8,si64,sial216",
                     help="comma-separated SCF workloads to time (sic8, si8, si64, sial216) or "
                          "'none'")
     ap.add_argument("--calib-copy", action="store_true",

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "sgpu_internal.h"
@@ -117,6 +118,104 @@ long gram_kslice(int n, long nd, int dtype)
     return std::min(ks, std::max(nd, 1L));
 }
 
+// ---- FP32 Gram products as BF16x3 on matrix cores -------------------------
+//
+// C = A' B over one K-slice per block row: every FP32 operand is split into
+// a BF16 head and a BF16 tail (a = a_hi + a_lo, a_lo = bf16(a - a_hi)) and
+// C += A_hi'B_hi + A_hi'B_lo + A_lo'B_hi on v_mfma_f32_16x16x16bf16_1k with
+// FP32 accumulation (the dropped A_lo'B_lo term is 2^-16 of the product);
+// the slices are summed in FP64 by sum_partials.  Block: 64 x 64 output
+// tile, 4 waves of 2 x 2 MFMA tiles; K staged through LDS 32 rows at a time,
+// coalesced along the columns, stored k-contiguous so that each lane's
+// A / B operand is one 8-byte LDS read.  Operand layout as in the U-Net
+// kernels: A(i,k) lane i + 16(k/4) element k%4, B(k,n) lane n + 16(k/4),
+// D(i,n) lane n + 16(i/4) register i%4.
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+constexpr int kGT = 64;        // output tile
+constexpr int kGK = 32;        // K rows per stage
+constexpr int kGLD = kGK + 4;  // LDS stride (elements) of a staged column
+
+__device__ __forceinline__ unsigned short bf16_bits(float f)
+{
+    unsigned u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+__global__ __launch_bounds__(256) void gram_bf16x3(const float *__restrict__ A, long lda,
+                                                   const float *__restrict__ B, long ldb, int n,
+                                                   long nd, long kslice, float *__restrict__ part)
+{
+    __shared__ __align__(16) unsigned short sa[2][kGT * kGLD], sb[2][kGT * kGLD];  // [hi/lo][col][k]
+    const int tilesn = (n + kGT - 1) / kGT;
+    const int ti = blockIdx.x % tilesn, tj = blockIdx.x / tilesn;
+    const int slice = blockIdx.y;
+    const long k0 = (long)slice * kslice, k1 = min(nd, k0 + kslice);
+    const int i0 = ti * kGT, j0 = tj * kGT;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, g = lane >> 4;
+    const int wi = (wave >> 1) * 32, wj = (wave & 1) * 32;
+    f32x4_t acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    for (long kb = k0; kb < k1; kb += kGK) {
+        // stage: element e = (col, k) with k fastest across threads (coalesced)
+#pragma unroll
+        for (int r = 0; r < kGT * kGK / 256; r++) {
+            const int e = threadIdx.x + r * 256;
+            const int kk = e % kGK, c = e / kGK;
+            const long k = kb + kk;
+            float va = 0.f, vb = 0.f;
+            if (k < k1) {
+                if (i0 + c < n) va = A[(long)(i0 + c) * lda + k];
+                if (j0 + c < n) vb = B[(long)(j0 + c) * ldb + k];
+            }
+            const unsigned short ha = bf16_bits(va), hb = bf16_bits(vb);
+            sa[0][c * kGLD + kk] = ha;
+            sa[1][c * kGLD + kk] = bf16_bits(va - __uint_as_float((unsigned)ha << 16));
+            sb[0][c * kGLD + kk] = hb;
+            sb[1][c * kGLD + kk] = bf16_bits(vb - __uint_as_float((unsigned)hb << 16));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < kGK; ks += 16) {
+            bf16x4_t ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                const int ca = (wi + t * 16 + li) * kGLD + ks + 4 * g;
+                const int cb = (wj + t * 16 + li) * kGLD + ks + 4 * g;
+                ah[t] = *reinterpret_cast<const bf16x4_t *>(&sa[0][ca]);
+                al[t] = *reinterpret_cast<const bf16x4_t *>(&sa[1][ca]);
+                bh[t] = *reinterpret_cast<const bf16x4_t *>(&sb[0][cb]);
+                bl[t] = *reinterpret_cast<const bf16x4_t *>(&sb[1][cb]);
+            }
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[a], bh[b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[a], bl[b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[a], bh[b], acc[a][b], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+    float *pp = part + (long)slice * n * n;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int i = i0 + wi + a * 16 + 4 * g + r;
+                const int j = j0 + wj + b * 16 + li;
+                if (i < n && j < n) pp[(long)j * n + i] = acc[a][b][r];
+            }
+}
+
 template <typename T>
 rocblas_status gemm_tn_batched(rocblas_handle h, int n, long k, const T *A, long lda, long sa,
                                const T *B, long ldb, long sb, T *C, long sc, int batch);
@@ -141,6 +240,15 @@ rocblas_status gemm_tn_batched<float>(rocblas_handle h, int n, long k, const flo
                                          (rocblas_int)ldb, sb, &zero, C, n, sc, batch);
 }
 
+bool gram_use_rocblas()
+{
+    static const int v = [] {
+        const char *e = getenv("SPARC_AMD_GRAM");
+        return (e && !strcmp(e, "rocblas")) ? 1 : 0;
+    }();
+    return v != 0;
+}
+
 template <typename T>
 int gram_tn_typed(sgpu_context *ctx, int n, long nd, const T *A, long lda, const T *B, long ldb,
                   double *C)
@@ -151,6 +259,15 @@ int gram_tn_typed(sgpu_context *ctx, int n, long nd, const T *A, long lda, const
     void *buf = nullptr;
     SGPU_CHECK(sgpu_scratch(ctx, (size_t)nslice * nn * sizeof(T), &buf));
     T *part = (T *)buf;
+    if (sizeof(T) == 4 && !gram_use_rocblas()) {
+        const int tilesn = (n + kGT - 1) / kGT;
+        gram_bf16x3<<<dim3(tilesn * tilesn, nslice), 256, 0, ctx->stream>>>(
+            (const float *)A, lda, (const float *)B, ldb, n, nd, kslice, (float *)part);
+        SGPU_HIP(hipGetLastError());
+        sum_partials<T><<<(unsigned)((nn + 255) / 256), 256, 0, ctx->stream>>>(part, nslice, nn, C);
+        SGPU_HIP(hipGetLastError());
+        return SGPU_OK;
+    }
     SGPU_BLAS(rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host));
     const long full = nd / kslice;
     if (full > 0)

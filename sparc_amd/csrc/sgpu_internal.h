@@ -111,7 +111,8 @@ struct sgpu_context {
     double d2x[SGPU_MAX_FDN + 1] = {0}, d2y[SGPU_MAX_FDN + 1] = {0}, d2z[SGPU_MAX_FDN + 1] = {0};
     // radius-6 tile shape (TX x TY, RY rows per thread): 0 = 32x16x2 (default,
     // fastest measured on MI355X), 1 = 32x32x4, 2 = 64x16x4, 3 = 32x32x8,
-    // 4 = 64x32x8; set from the SPARC_AMD_STENCIL_TILE environment variable
+    // 4 = 64x32x8, 5 / 6 = 32x16x2 / 32x8x2 with an 8-plane LDS ring (FP32;
+    // FP64 uses 0); set from the SPARC_AMD_STENCIL_TILE environment variable
     // at creation
     int stencil_cfg = 0;
     hipStream_t stream = nullptr;

@@ -233,6 +233,165 @@ __global__ __launch_bounds__(TX *(TY / RY)) void stencil_tiled(StencilParams<T> 
     }
 }
 
+// Ring variant: an NS-slot LDS ring of whole planes (interior + halo).  Plane
+// z + kR is fetched kR steps before its x/y use -- interior into the
+// register z-queue and the ring, halo into the ring -- i.e. at the moment the
+// neighbouring tiles fetch the same rows as their interior, so halo rows are
+// served from L2 instead of HBM.  One barrier per step: the slot written at
+// step z, (z + kR) % NS, was last read at step z + kR - NS <= z - 2, and
+// every thread has passed step z - 1's barrier since then.
+template <typename T, int TX, int TY, int RY, int NS>
+__global__ __launch_bounds__(TX *(TY / RY)) void stencil_ring(StencilParams<T> p, int zchunk)
+{
+    static_assert(NS >= kR + 2, "ring too short for the one-barrier schedule");
+    constexpr int NTY = TY / RY;
+    constexpr int NT = TX * NTY;
+    constexpr int LX = TX + 2 * kR;
+    constexpr int LY = TY + 2 * kR;
+    constexpr int NHALO = 2 * kR * TY + 2 * kR * TX;
+    constexpr int HPT = (NHALO + NT - 1) / NT;
+    __shared__ T ring[NS][LY][LX];
+
+    const int tilesx = (p.nx + TX - 1) / TX;
+    const int lid = xcd_remap(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z),
+                              gridDim.x * gridDim.y * gridDim.z);
+    const int tid_xy = lid % gridDim.x, rest = lid / gridDim.x;
+    const int bx = tid_xy % tilesx;
+    const int by = tid_xy / tilesx;
+    const int col = rest % gridDim.y;
+    const int zs = (rest / gridDim.y) * zchunk;
+    const int ze = min(p.nz, zs + zchunk);
+    const int tx = threadIdx.x % TX;
+    const int tyb = threadIdx.x / TX;
+    const int i0 = bx * TX, j0 = by * TY;
+    const long nxny = (long)p.nx * p.ny;
+    const T *x = p.x + col * p.ldx;
+
+    int coff[RY], ooff[RY];
+#pragma unroll
+    for (int r = 0; r < RY; r++) {
+        const int ty = tyb + r * NTY;
+        const int ig = i0 + tx, jg = j0 + ty;
+        const int ia = axis_index(ig, p.nx, p.bcx), ja = axis_index(jg, p.ny, p.bcy);
+        coff[r] = (ia >= 0 && ja >= 0) ? ja * p.nx + ia : -1;
+        ooff[r] = (ig < p.nx && jg < p.ny) ? jg * p.nx + ig : -1;
+    }
+    int hslot[HPT], hoff[HPT];
+#pragma unroll
+    for (int h = 0; h < HPT; h++) {
+        const int e = threadIdx.x + h * NT;
+        hslot[h] = -1;
+        hoff[h] = -1;
+        if (e < NHALO) {
+            int ly, lx;
+            if (e < 2 * kR * TY) {
+                const int row = e / (2 * kR), c = e % (2 * kR);
+                ly = kR + row;
+                lx = c < kR ? c : TX + c;
+            } else {
+                const int e2 = e - 2 * kR * TY;
+                const int rr = e2 / TX, c = e2 % TX;
+                ly = rr < kR ? rr : TY + rr;
+                lx = kR + c;
+            }
+            hslot[h] = ly * LX + lx;
+            const int ia = axis_index(i0 - kR + lx, p.nx, p.bcx);
+            const int ja = axis_index(j0 - kR + ly, p.ny, p.bcy);
+            hoff[h] = (ia >= 0 && ja >= 0) ? ja * p.nx + ia : -1;
+        }
+    }
+    auto plane = [&](int zz) -> const T * {
+        const int za = axis_index(zz, p.nz, p.bcz);
+        return za < 0 ? nullptr : x + (long)za * nxny;
+    };
+
+    // register queue: q[r][s + 1] = plane zs - kR + s; ring slots for planes
+    // zs .. zs + kR - 1 (interior from the queue, halo fetched here)
+    T q[RY][2 * kR + 1];
+#pragma unroll
+    for (int s = 0; s < 2 * kR; s++) {
+        const T *xa = plane(zs - kR + s);
+#pragma unroll
+        for (int r = 0; r < RY; r++) q[r][s + 1] = (xa && coff[r] >= 0) ? xa[coff[r]] : T(0);
+    }
+#pragma unroll
+    for (int t = 0; t < kR; t++) {
+        T(*sl)[LX] = ring[(zs + t) % NS];
+#pragma unroll
+        for (int r = 0; r < RY; r++) sl[kR + tyb + r * NTY][kR + tx] = q[r][kR + t + 1];
+        const T *xa = plane(zs + t);
+#pragma unroll
+        for (int h = 0; h < HPT; h++)
+            if (hslot[h] >= 0) (&sl[0][0])[hslot[h]] = (xa && hoff[h] >= 0) ? xa[hoff[h]] : T(0);
+    }
+    // prefetch of plane zs + kR (interior and halo)
+    T nq[RY], nh[HPT];
+    {
+        const T *xn = plane(zs + kR);
+#pragma unroll
+        for (int r = 0; r < RY; r++) nq[r] = (xn && coff[r] >= 0) ? xn[coff[r]] : T(0);
+#pragma unroll
+        for (int h = 0; h < HPT; h++) nh[h] = (xn && hoff[h] >= 0) ? xn[hoff[h]] : T(0);
+    }
+    for (int z = zs; z < ze; z++) {
+#pragma unroll
+        for (int r = 0; r < RY; r++) {
+#pragma unroll
+            for (int s = 0; s < 2 * kR; s++) q[r][s] = q[r][s + 1];
+            q[r][2 * kR] = nq[r];
+        }
+        {
+            T(*sl)[LX] = ring[(z + kR) % NS];
+#pragma unroll
+            for (int r = 0; r < RY; r++) sl[kR + tyb + r * NTY][kR + tx] = nq[r];
+#pragma unroll
+            for (int h = 0; h < HPT; h++)
+                if (hslot[h] >= 0) (&sl[0][0])[hslot[h]] = nh[h];
+        }
+        if (z + 1 < ze) {
+            const T *xn = plane(z + 1 + kR);
+#pragma unroll
+            for (int r = 0; r < RY; r++) nq[r] = (xn && coff[r] >= 0) ? xn[coff[r]] : T(0);
+#pragma unroll
+            for (int h = 0; h < HPT; h++) nh[h] = (xn && hoff[h] >= 0) ? xn[hoff[h]] : T(0);
+        }
+        __syncthreads();
+        T(*tl)[LX] = ring[z % NS];
+        const long zb = (long)z * nxny;
+#pragma unroll
+        for (int r = 0; r < RY; r++) {
+            if (ooff[r] < 0) continue;
+            const int ly = kR + tyb + r * NTY, lx = kR + tx;
+            const T xc = q[r][kR];
+            T res = p.w0 * xc;
+#pragma unroll
+            for (int s = 1; s <= kR; s++) {
+                const T rx = (tl[ly][lx - s] + tl[ly][lx + s]) * p.wx[s];
+                const T ry = (tl[ly - s][lx] + tl[ly + s][lx]) * p.wy[s];
+                const T rz = (q[r][kR - s] + q[r][kR + s]) * p.wz[s];
+                res += rx + ry + rz;
+            }
+            const long o = zb + ooff[r];
+            if (p.bdiag != T(0)) res += p.bdiag * (p.v[o] * xc);
+            T val = p.s1 * res;
+            if (p.z) val += p.s2 * p.z[col * p.ldz + o];
+            p.out[col * p.ldo + o] = val;
+        }
+    }
+}
+
+template <typename T, int TX, int TY, int RY, int NS>
+void launch_ring(sgpu_context *ctx, const StencilParams<T> &p, int ncol)
+{
+    const int tiles = ((ctx->nx + TX - 1) / TX) * ((ctx->ny + TY - 1) / TY);
+    long want = (2048 + (long)tiles * ncol - 1) / ((long)tiles * ncol);
+    int nzc = (int)std::max(1L, std::min<long>(want, std::max(1, ctx->nz / 16)));
+    int zchunk = (ctx->nz + nzc - 1) / nzc;
+    nzc = (ctx->nz + zchunk - 1) / zchunk;
+    dim3 grid(tiles, ncol, nzc);
+    stencil_ring<T, TX, TY, RY, NS><<<grid, TX * (TY / RY), 0, ctx->stream>>>(p, zchunk);
+}
+
 template <typename T, int TX, int TY, int RY>
 void launch_tiled(sgpu_context *ctx, const StencilParams<T> &p, int ncol)
 {
@@ -276,6 +435,14 @@ int launch_typed(sgpu_context *ctx, int ncol, double a, double bdiag, double c, 
     }
     if (allow_tiled && ctx->fdn == kR) {
         switch (ctx->stencil_cfg) {
+            case 5:
+                if (sizeof(T) == 4) launch_ring<T, 32, 16, 2, 8>(ctx, p, ncol);
+                else launch_tiled<T, 32, 16, 2>(ctx, p, ncol);
+                break;
+            case 6:
+                if (sizeof(T) == 4) launch_ring<T, 32, 8, 2, 8>(ctx, p, ncol);
+                else launch_tiled<T, 32, 16, 2>(ctx, p, ncol);
+                break;
             case 1: launch_tiled<T, 32, 32, 4>(ctx, p, ncol); break;
             case 2: launch_tiled<T, 64, 16, 4>(ctx, p, ncol); break;
             case 3: launch_tiled<T, 32, 32, 8>(ctx, p, ncol); break;

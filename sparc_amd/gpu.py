@@ -369,6 +369,9 @@ class Context:
         return emin.value, emax.value, it.value
 
     # -- device-resident operations
+    def lap_plus_diag_dev(self, a: float, b: float, c: float, x: Block, y: Block):
+        self._check(self.lib.sgpu_lap_plus_diag_dev(self.h, a, b, c, x.h, y.h))
+
     def hamiltonian_dev(self, c: float, x: Block, Hx: Block):
         self._check(self.lib.sgpu_hamiltonian_dev(self.h, c, x.h, Hx.h))
 

@@ -49,6 +49,28 @@ def test_stencil_tile_shapes(gpu, oracle_mod, monkeypatch, cfg, n, bc):
     assert rel_err(y, o.lap_plus_diag(x, -0.5, 1.0, 0.4, veff)) < 1e-13
 
 
+@pytest.mark.parametrize("cfg", [5, 6])
+@pytest.mark.parametrize("n,bc,ncol", [((40, 35, 13), (0, 0, 0), 3), ((70, 33, 9), (0, 1, 1), 2),
+                                       ((7, 6, 30), (0, 0, 0), 1), ((33, 17, 40), (1, 0, 0), 1)])
+def test_stencil_ring_fp32(gpu, oracle_mod, monkeypatch, cfg, n, bc, ncol):
+    """The LDS-ring stencil (FP32 blocks) against the oracle: partial tiles,
+    wrap, walls, and z-chunked single columns."""
+    monkeypatch.setenv("SPARC_AMD_STENCIL_TILE", str(cfg))
+    g, veff, nl, rng = small_case(n=n, bc=bc, seed=40 + cfg)
+    o = oracle_mod.Oracle(g, veff)
+    x = rng.uniform(-0.5, 0.5, (g.nd, ncol))
+    with gpu.Context(g) as ctx:
+        ctx.set_veff(veff)
+        bx = ctx.block(ncol, gpu.F32).upload(x)
+        by = ctx.block(ncol, gpu.F32)
+        ctx.lap_plus_diag_dev(-0.5, 1.0, 0.4, bx, by)
+        y = by.download()
+        y64 = ctx.Lap_plus_diag_vec_mult(x, -0.5, 1.0, 0.4, veff)
+    x32 = x.astype(np.float32).astype(np.float64)
+    assert rel_err(y, o.lap_plus_diag(x32, -0.5, 1.0, 0.4, veff)) < 3e-6
+    assert rel_err(y64, o.lap_plus_diag(x, -0.5, 1.0, 0.4, veff)) < 1e-13
+
+
 @pytest.mark.parametrize("order", [2, 4, 8, 16])
 def test_lap_other_orders(gpu, oracle_mod, order):
     g, veff, nl, rng = small_case(n=(12, 11, 10), order=order, seed=2)
