@@ -118,18 +118,20 @@ long gram_kslice(int n, long nd, int dtype)
     return std::min(ks, std::max(nd, 1L));
 }
 
-// ---- FP32 Gram products as BF16x3 on matrix cores -------------------------
+// ---- FP32 Gram products from BF16 pieces on matrix cores ------------------
 //
-// C = A' B over one K-slice per block row: every FP32 operand is split into
-// a BF16 head and a BF16 tail (a = a_hi + a_lo, a_lo = bf16(a - a_hi)) and
-// C += A_hi'B_hi + A_hi'B_lo + A_lo'B_hi on v_mfma_f32_16x16x16bf16_1k with
-// FP32 accumulation (the dropped A_lo'B_lo term is 2^-16 of the product);
-// the slices are summed in FP64 by sum_partials.  Block: 64 x 64 output
-// tile, 4 waves of 2 x 2 MFMA tiles; K staged through LDS 32 rows at a time,
-// coalesced along the columns, stored k-contiguous so that each lane's
-// A / B operand is one 8-byte LDS read.  Operand layout as in the U-Net
-// kernels: A(i,k) lane i + 16(k/4) element k%4, B(k,n) lane n + 16(k/4),
-// D(i,n) lane n + 16(i/4) register i%4.
+// C = A' B over one K-slice per block row.  Every FP32 operand is split into
+// three BF16 pieces, a = a0 + a1 + a2 (a1 = bf16(a - a0), a2 = bf16(a - a0 -
+// a1): 24 significant bits, the FP32 significand), and
+//   C += A0'B0 + A0'B1 + A1'B0 + A0'B2 + A2'B0 + A1'B1
+// on v_mfma_f32_16x16x16bf16_1k with FP32 accumulation; the dropped products
+// are below 2^-24 of |a||b|.  (Two pieces, three products, carry only ~17
+// bits and stalled the SiC SCF above TOL_SCF = 1e-6.)  The slices are summed
+// in FP64 by sum_partials.  Block: 64 x 64 output tile, 4 waves of 2 x 2 MFMA
+// tiles; K staged through LDS 32 rows at a time, coalesced along the
+// columns, stored k-contiguous so that each lane's operand is one 8-byte LDS
+// read.  Operand layout as in the U-Net kernels: A(i,k) lane i + 16(k/4)
+// element k%4, B(k,n) lane n + 16(k/4), D(i,n) lane n + 16(i/4) register i%4.
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 constexpr int kGT = 64;        // output tile
@@ -142,12 +144,13 @@ __device__ __forceinline__ unsigned short bf16_bits(float f)
     u += 0x7fffu + ((u >> 16) & 1u);
     return (unsigned short)(u >> 16);
 }
+__device__ __forceinline__ float bf16_val(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
 
 __global__ __launch_bounds__(256) void gram_bf16x3(const float *__restrict__ A, long lda,
                                                    const float *__restrict__ B, long ldb, int n,
                                                    long nd, long kslice, float *__restrict__ part)
 {
-    __shared__ __align__(16) unsigned short sa[2][kGT * kGLD], sb[2][kGT * kGLD];  // [hi/lo][col][k]
+    __shared__ __align__(16) unsigned short sa[3][kGT * kGLD], sb[3][kGT * kGLD];  // [piece][col][k]
     const int tilesn = (n + kGT - 1) / kGT;
     const int ti = blockIdx.x % tilesn, tj = blockIdx.x / tilesn;
     const int slice = blockIdx.y;
@@ -173,32 +176,42 @@ __global__ __launch_bounds__(256) void gram_bf16x3(const float *__restrict__ A, 
                 if (i0 + c < n) va = A[(long)(i0 + c) * lda + k];
                 if (j0 + c < n) vb = B[(long)(j0 + c) * ldb + k];
             }
-            const unsigned short ha = bf16_bits(va), hb = bf16_bits(vb);
-            sa[0][c * kGLD + kk] = ha;
-            sa[1][c * kGLD + kk] = bf16_bits(va - __uint_as_float((unsigned)ha << 16));
-            sb[0][c * kGLD + kk] = hb;
-            sb[1][c * kGLD + kk] = bf16_bits(vb - __uint_as_float((unsigned)hb << 16));
+            const int o = c * kGLD + kk;
+            unsigned short p0 = bf16_bits(va), p1 = bf16_bits(va - bf16_val(p0));
+            sa[0][o] = p0;
+            sa[1][o] = p1;
+            sa[2][o] = bf16_bits(va - bf16_val(p0) - bf16_val(p1));
+            p0 = bf16_bits(vb);
+            p1 = bf16_bits(vb - bf16_val(p0));
+            sb[0][o] = p0;
+            sb[1][o] = p1;
+            sb[2][o] = bf16_bits(vb - bf16_val(p0) - bf16_val(p1));
         }
         __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < kGK; ks += 16) {
-            bf16x4_t ah[2], al[2], bh[2], bl[2];
+            bf16x4_t av[3][2], bv[3][2];
 #pragma unroll
             for (int t = 0; t < 2; t++) {
                 const int ca = (wi + t * 16 + li) * kGLD + ks + 4 * g;
                 const int cb = (wj + t * 16 + li) * kGLD + ks + 4 * g;
-                ah[t] = *reinterpret_cast<const bf16x4_t *>(&sa[0][ca]);
-                al[t] = *reinterpret_cast<const bf16x4_t *>(&sa[1][ca]);
-                bh[t] = *reinterpret_cast<const bf16x4_t *>(&sb[0][cb]);
-                bl[t] = *reinterpret_cast<const bf16x4_t *>(&sb[1][cb]);
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    av[q][t] = *reinterpret_cast<const bf16x4_t *>(&sa[q][ca]);
+                    bv[q][t] = *reinterpret_cast<const bf16x4_t *>(&sb[q][cb]);
+                }
             }
 #pragma unroll
             for (int a = 0; a < 2; a++)
 #pragma unroll
                 for (int b = 0; b < 2; b++) {
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[a], bh[b], acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[a], bl[b], acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[a], bh[b], acc[a][b], 0, 0, 0);
+                    // smallest products first
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[1][a], bv[1][b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[0][a], bv[2][b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[2][a], bv[0][b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[0][a], bv[1][b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[1][a], bv[0][b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[0][a], bv[0][b], acc[a][b], 0, 0, 0);
                 }
         }
         __syncthreads();
@@ -259,7 +272,10 @@ int gram_tn_typed(sgpu_context *ctx, int n, long nd, const T *A, long lda, const
     void *buf = nullptr;
     SGPU_CHECK(sgpu_scratch(ctx, (size_t)nslice * nn * sizeof(T), &buf));
     T *part = (T *)buf;
-    if (sizeof(T) == 4 && !gram_use_rocblas()) {
+    // BF16-piece kernel for Ns < 256; above that its 64 x 64 output tiles
+    // re-read Y about 16 times and rocBLAS's FP32 GEMM was measured faster
+    // (474k x 480: 5.2 vs 11.9 ms per SCF iteration with the 2-piece variant)
+    if (sizeof(T) == 4 && n < 256 && !gram_use_rocblas()) {
         const int tilesn = (n + kGT - 1) / kGT;
         gram_bf16x3<<<dim3(tilesn * tilesn, nslice), 256, 0, ctx->stream>>>(
             (const float *)A, lda, (const float *)B, ldb, n, nd, kslice, (float *)part);

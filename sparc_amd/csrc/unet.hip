@@ -16,6 +16,7 @@
 // Anderson mixing assumes.  Activations are stored channels-last
 // ([voxel][C], 32 bytes per voxel) in BF16; products are BF16 x BF16 (weights
 // rounded to BF16 when staged in LDS) accumulated in FP32.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -179,6 +180,7 @@ __global__ __launch_bounds__(256) void convT_cc(const uint16_t *__restrict__ in,
 // register i%4.  One wave per 16-voxel tile, four tiles per block.
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int kTilesPerWave = 8;  // 16-voxel tiles per wave: amortises the weight staging
 
 // weights W[co][ci][27] -> LDS [k][co][ci] as BF16 bits
 __device__ void stage_weights_bf16(const float *__restrict__ W, uint16_t *lds)
@@ -201,32 +203,44 @@ template <int S>
 __global__ __launch_bounds__(256) void conv_cc_mfma(const uint16_t *__restrict__ in, Dims di,
                                                     uint16_t *__restrict__ out, Dims dout,
                                                     const float *__restrict__ W,
-                                                    const uint16_t *__restrict__ skip)
+                                                    const uint16_t *__restrict__ skip, int tpw)
 {
     __shared__ __align__(16) uint16_t wl[27 * kC * kC];
     stage_weights_bf16(W, wl);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, g = lane >> 4;
     const int txn = (dout.nx + 15) / 16;
-    const long tile = (long)blockIdx.x * 4 + wave;
-    if (tile >= (long)txn * dout.ny * dout.nz) return;
-    const int tx = (int)(tile % txn);
-    const long row = tile / txn;
-    const int oy = (int)(row % dout.ny), oz = (int)(row / dout.ny);
-    const int ox = min(tx * 16 + li, dout.nx - 1);  // A row voxel of this lane (clamped)
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k = 0; k < 27; k++) {
-        const int ix = wrap(S * ox + k % 3 - 1, di.nx);
-        const int iy = wrap(S * oy + (k / 3) % 3 - 1, di.ny);
-        const int iz = wrap(S * oz + k / 9 - 1, di.nz);
-        const bf16x4 a = *reinterpret_cast<const bf16x4 *>(in + di.vox(ix, iy, iz) * kC + 4 * g);
-        const bf16x4 b = *reinterpret_cast<const bf16x4 *>(wl + (k * kC + li) * kC + 4 * g);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-    }
+    const long ntiles = (long)txn * dout.ny * dout.nz;
+    for (int t = 0; t < tpw; t++) {
+        const long tile = ((long)blockIdx.x * tpw + t) * 4 + wave;
+        if (tile >= ntiles) return;
+        const int tx = (int)(tile % txn);
+        const long row = tile / txn;
+        const int oy = (int)(row % dout.ny), oz = (int)(row / dout.ny);
+        const int ox = min(tx * 16 + li, dout.nx - 1);  // A row voxel of this lane (clamped)
+        // all 27 tap loads issued before the dependent MFMA chain
+        int xs[3], ys[3], zs[3];
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int x = tx * 16 + 4 * g + j;
-        if (x < dout.nx) store_tile(out, skip, dout.vox(x, oy, oz), li, acc[j]);
+        for (int d = 0; d < 3; d++) {
+            xs[d] = wrap(S * ox + d - 1, di.nx);
+            ys[d] = wrap(S * oy + d - 1, di.ny);
+            zs[d] = wrap(S * oz + d - 1, di.nz);
+        }
+        bf16x4 av[27];
+#pragma unroll
+        for (int k = 0; k < 27; k++)
+            av[k] = *reinterpret_cast<const bf16x4 *>(in + di.vox(xs[k % 3], ys[(k / 3) % 3], zs[k / 9]) * kC + 4 * g);
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < 27; k++) {
+            const bf16x4 b = *reinterpret_cast<const bf16x4 *>(wl + (k * kC + li) * kC + 4 * g);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[k], b, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int x = tx * 16 + 4 * g + j;
+            if (x < dout.nx) store_tile(out, skip, dout.vox(x, oy, oz), li, acc[j]);
+        }
     }
 }
 
@@ -235,15 +249,17 @@ __global__ __launch_bounds__(256) void conv_cc_mfma(const uint16_t *__restrict__
 __global__ __launch_bounds__(256) void convT_cc_mfma(const uint16_t *__restrict__ in, Dims di,
                                                      uint16_t *__restrict__ out, Dims dout,
                                                      const float *__restrict__ W,
-                                                     const uint16_t *__restrict__ skip)
+                                                     const uint16_t *__restrict__ skip, int tpw)
 {
     __shared__ __align__(16) uint16_t wl[27 * kC * kC];
     stage_weights_bf16(W, wl);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, g = lane >> 4;
     const int txn = (di.nx + 15) / 16;
-    const long tile = (long)blockIdx.x * 4 + wave;
-    if (tile >= (long)txn * 2 * dout.ny * dout.nz) return;
+    const long ntiles = (long)txn * 2 * dout.ny * dout.nz;
+    for (int tt = 0; tt < tpw; tt++) {
+    const long tile = ((long)blockIdx.x * tpw + tt) * 4 + wave;
+    if (tile >= ntiles) return;
     const int tx = (int)(tile % txn);
     long rest = tile / txn;
     const int px = (int)(rest % 2);
@@ -257,20 +273,31 @@ __global__ __launch_bounds__(256) void convT_cc_mfma(const uint16_t *__restrict_
     else { ky[0] = 1; cy[0] = fy / 2; }
     if (fz & 1) { kz[0] = 0; cz[0] = wrap((fz + 1) / 2, di.nz); kz[1] = 2; cz[1] = (fz - 1) / 2; }
     else { kz[0] = 1; cz[0] = fz / 2; }
+    // up to 8 taps: loads first, then the MFMA chain
+    bf16x4 av[8];
+    int kk[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        const int a = t >> 2, b = (t >> 1) & 1, e = t & 1;
+        kk[t] = -1;
+        if (a < nkz && b < nky && e < nkx) {
+            kk[t] = kz[a] * 9 + ky[b] * 3 + kx[e];
+            const int cx = wrap(c + ox[e], di.nx);
+            av[t] = *reinterpret_cast<const bf16x4 *>(in + di.vox(cx, cy[b], cz[a]) * kC + 4 * g);
+        }
+    }
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int a = 0; a < nkz; a++)
-        for (int b = 0; b < nky; b++)
-            for (int e = 0; e < nkx; e++) {
-                const int k = kz[a] * 9 + ky[b] * 3 + kx[e];
-                const int cx = wrap(c + ox[e], di.nx);
-                const bf16x4 av = *reinterpret_cast<const bf16x4 *>(in + di.vox(cx, cy[b], cz[a]) * kC + 4 * g);
-                const bf16x4 bv = *reinterpret_cast<const bf16x4 *>(wl + (k * kC + li) * kC + 4 * g);
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
-            }
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        if (kk[t] < 0) continue;
+        const bf16x4 bv = *reinterpret_cast<const bf16x4 *>(wl + (kk[t] * kC + li) * kC + 4 * g);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[t], bv, acc, 0, 0, 0);
+    }
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const int cc = tx * 16 + 4 * g + j;
         if (cc < di.nx) store_tile(out, skip, dout.vox(2 * cc + px, fy, fz), li, acc[j]);
+    }
     }
 }
 
@@ -498,6 +525,13 @@ unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 
 bool g_unet_scalar = false;  // SPARC_AMD_UNET_SCALAR=1 selects the FP32-FMA layer kernels
 
+// 16-voxel tiles per wave: enough to amortise the weight staging, few enough
+// to keep ~2048 blocks in flight (the quarter-resolution layers are small)
+int tiles_per_wave(long tiles)
+{
+    return (int)std::max(1L, std::min<long>(kTilesPerWave, tiles / (4L * 2048)));
+}
+
 template <int S>
 void launch_conv_cc(const uint16_t *in, Dims di, uint16_t *out, Dims dout, const float *W,
                     const uint16_t *skip, hipStream_t s)
@@ -506,7 +540,10 @@ void launch_conv_cc(const uint16_t *in, Dims di, uint16_t *out, Dims dout, const
         conv_cc<S><<<nblk(dout.n()), 256, 0, s>>>(in, di, out, dout, W, skip);
     } else {
         const long tiles = (long)((dout.nx + 15) / 16) * dout.ny * dout.nz;
-        conv_cc_mfma<S><<<(unsigned)((tiles + 3) / 4), 256, 0, s>>>(in, di, out, dout, W, skip);
+        const int tpw = tiles_per_wave(tiles);
+        const long per = 4L * tpw;
+        conv_cc_mfma<S><<<(unsigned)((tiles + per - 1) / per), 256, 0, s>>>(in, di, out, dout, W, skip,
+                                                                         tpw);
     }
 }
 
@@ -517,7 +554,10 @@ void launch_convT_cc(const uint16_t *in, Dims di, uint16_t *out, Dims dout, cons
         convT_cc<<<nblk(dout.n()), 256, 0, s>>>(in, di, out, dout, W, skip);
     } else {
         const long tiles = (long)((di.nx + 15) / 16) * 2 * dout.ny * dout.nz;
-        convT_cc_mfma<<<(unsigned)((tiles + 3) / 4), 256, 0, s>>>(in, di, out, dout, W, skip);
+        const int tpw = tiles_per_wave(tiles);
+        const long per = 4L * tpw;
+        convT_cc_mfma<<<(unsigned)((tiles + per - 1) / per), 256, 0, s>>>(in, di, out, dout, W, skip,
+                                                                         tpw);
     }
 }
 

@@ -167,3 +167,21 @@ def test_generalized_eigensolve_filtered_scale(gpu):
     lam_ref = sla.eigh(Hp, Mp, eigvals_only=True, driver="gvd")
     assert np.max(np.abs(lam - lam_ref)) < 1e-10
     assert np.max(np.abs(Q.T @ Mp @ Q - np.eye(n))) < 1e-9
+
+
+@pytest.mark.parametrize("ns", [24, 160, 300])
+def test_fp32_gram_accuracy(gpu, ns):
+    """The FP32 projection (BF16-piece MFMA kernel below 256 states, rocBLAS
+    above) against FP64 products of the same FP32 inputs: FP32-level error
+    (5e-6 of max |Mp|), which the SCF at TOL_SCF = 1e-6 needs."""
+    from sparc_amd.workloads import Grid
+    g = Grid(20, 20, 20, 0.5, 0.5, 0.5)
+    rng = np.random.default_rng(ns)
+    Y = rng.uniform(-0.5, 0.5, (g.nd, ns)) * np.exp(rng.uniform(-3, 3, ns))
+    Y32 = Y.astype(np.float32).astype(np.float64)
+    veff = np.zeros(g.nd)
+    with gpu.Context(g) as ctx:
+        ctx.set_veff(veff)
+        Hp, Mp, _ = ctx.DP_Project_Hamiltonian(Y, gpu.F32)
+    ref = Y32.T @ Y32
+    assert np.max(np.abs(Mp - ref)) < 5e-6 * np.max(np.abs(ref))
