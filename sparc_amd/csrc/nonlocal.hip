@@ -129,18 +129,33 @@ __global__ __launch_bounds__(256) void nlb_project(const T *__restrict__ X, long
         for (int rt = 0; rt < kRT; rt++)
 #pragma unroll
             for (int ct = 0; ct < NCT; ct++) acc[rt][ct] = typename M::acc{0, 0, 0, 0};
-        for (int k = qs; k < qs + kBP / kWaves; k += 4) {
-            const int q = k + lk;
-            T bv[NCT];
+        // kU k-steps at a time: their Phi loads are all issued before the
+        // MFMAs that consume them (one dependent global load per MFMA step
+        // left the kernel latency-bound at ~1 TB/s)
+        constexpr int kU = 8;
+        for (int k = qs; k < qs + kBP / kWaves; k += 4 * kU) {
+            T av[kU][kRT];
 #pragma unroll
-            for (int ct = 0; ct < NCT; ct++) bv[ct] = xs[(ct * 16 + li) * kLDQ + q];
+            for (int u = 0; u < kU; u++) {
+                const int q = k + 4 * u + lk;
 #pragma unroll
-            for (int rt = 0; rt < kRT; rt++) {
-                if (j0 + rt * 16 < np) {
+                for (int rt = 0; rt < kRT; rt++) {
                     const int j = j0 + rt * 16 + li;
-                    const T a = j < np ? ph[(long)q * np + j] : T(0);
+                    av[u][rt] = (j0 + rt * 16 < np && j < np) ? ph[(long)q * np + j] : T(0);
+                }
+            }
 #pragma unroll
-                    for (int ct = 0; ct < NCT; ct++) acc[rt][ct] = M::run(a, bv[ct], acc[rt][ct]);
+            for (int u = 0; u < kU; u++) {
+                const int q = k + 4 * u + lk;
+                T bv[NCT];
+#pragma unroll
+                for (int ct = 0; ct < NCT; ct++) bv[ct] = xs[(ct * 16 + li) * kLDQ + q];
+#pragma unroll
+                for (int rt = 0; rt < kRT; rt++) {
+                    if (j0 + rt * 16 < np) {
+#pragma unroll
+                        for (int ct = 0; ct < NCT; ct++) acc[rt][ct] = M::run(av[u][rt], bv[ct], acc[rt][ct]);
+                    }
                 }
             }
         }
@@ -226,11 +241,23 @@ __global__ __launch_bounds__(256) void nlb_apply(T *__restrict__ out, long ldo, 
         typename M::acc acc[NCT];
 #pragma unroll
         for (int ct = 0; ct < NCT; ct++) acc[ct] = typename M::acc{0, 0, 0, 0};
-        for (int j = 0; j < np; j += 4) {
-            const T bq = pc[(long)(j + lk) * kBP + q0 + li];  // B (j, point)
+        // Phi loads of up to 8 j-steps issued before their MFMAs
+        for (int jb = 0; jb < np; jb += 32) {
+            T bqs[8];
 #pragma unroll
-            for (int ct = 0; ct < NCT; ct++)                   // A (col, j)
-                acc[ct] = M::run(beta[(j + lk) * LDB + ct * 16 + li], bq, acc[ct]);
+            for (int u = 0; u < 8; u++) {
+                const int j = jb + 4 * u;
+                bqs[u] = j < np ? pc[(long)(j + lk) * kBP + q0 + li] : T(0);  // B (j, point)
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int j = jb + 4 * u;
+                if (j < np) {
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ct++)                               // A (col, j)
+                        acc[ct] = M::run(beta[(j + lk) * LDB + ct * 16 + li], bqs[u], acc[ct]);
+                }
+            }
         }
         const long pt = brick_point(q0 + li, x0, y0, z0, nx, ny, nz);
         if (pt < 0) continue;
@@ -299,19 +326,28 @@ __global__ __launch_bounds__(256) void nls_project(const T *__restrict__ X, long
             xs[c * kQLD + q] = (g >= 0 && col0 + c < ncol) ? X[(long)(col0 + c) * ldx + g] : T(0);
         }
         __syncthreads();
-        const int qw = qn / 4;
-        for (int k = wave * qw; k < (wave + 1) * qw; k += 4) {
-            const int q = k + lk;
-            T bv[NCT];
+        const int qw = qn / 4;  // a multiple of 16
+        for (int k = wave * qw; k < (wave + 1) * qw; k += 16) {
+            // four k-steps: Chi loads first, then the MFMAs
+            T av[4][MRT];
 #pragma unroll
-            for (int ct = 0; ct < NCT; ct++) bv[ct] = xs[(ct * 16 + li) * kQLD + q];
-            const T *arow = ch + (long)(qc + q) * npad + li;
+            for (int u = 0; u < 4; u++) {
+                const T *arow = ch + (long)(qc + k + 4 * u + lk) * npad + li;
 #pragma unroll
-            for (int rt = 0; rt < MRT; rt++) {
-                if (rt < nrt) {
-                    const T a = arow[rt * 16];
+                for (int rt = 0; rt < MRT; rt++) av[u][rt] = rt < nrt ? arow[rt * 16] : T(0);
+            }
 #pragma unroll
-                    for (int ct = 0; ct < NCT; ct++) acc[rt][ct] = M::run(a, bv[ct], acc[rt][ct]);
+            for (int u = 0; u < 4; u++) {
+                const int q = k + 4 * u + lk;
+                T bv[NCT];
+#pragma unroll
+                for (int ct = 0; ct < NCT; ct++) bv[ct] = xs[(ct * 16 + li) * kQLD + q];
+#pragma unroll
+                for (int rt = 0; rt < MRT; rt++) {
+                    if (rt < nrt) {
+#pragma unroll
+                        for (int ct = 0; ct < NCT; ct++) acc[rt][ct] = M::run(av[u][rt], bv[ct], acc[rt][ct]);
+                    }
                 }
             }
         }
@@ -386,11 +422,18 @@ __global__ __launch_bounds__(256) void nls_apply(T *__restrict__ out, long ldo, 
         typename M::acc acc[NCT];
 #pragma unroll
         for (int ct = 0; ct < NCT; ct++) acc[ct] = typename M::acc{0, 0, 0, 0};
-        for (int j = 0; j < npad; j += 4) {
-            const T bq = ct_[(long)(j + lk) * qpad + q0 + li];  // B (p, point)
+        for (int jb = 0; jb < npad; jb += 16) {
+            // npad is a multiple of 16: four p-steps, Chi loads first
+            T bqs[4];
 #pragma unroll
-            for (int ct = 0; ct < NCT; ct++)                    // A (col, p)
-                acc[ct] = M::run(beta[(j + lk) * LDB + ct * 16 + li], bq, acc[ct]);
+            for (int u = 0; u < 4; u++) bqs[u] = ct_[(long)(jb + 4 * u + lk) * qpad + q0 + li];  // B (p, point)
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int j = jb + 4 * u;
+#pragma unroll
+                for (int ct = 0; ct < NCT; ct++)                    // A (col, p)
+                    acc[ct] = M::run(beta[(j + lk) * LDB + ct * 16 + li], bqs[u], acc[ct]);
+            }
         }
         const int g = gpos[poff + q0 + li];
         if (g < 0) continue;
