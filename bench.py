@@ -275,6 +275,9 @@ def main(argv=None):
     ap.add_argument("--scf", default="si8,si64,sial216",
                     help="comma-separated SCF workloads to time (sic8, si8, si64, sial216) or "
                          "'none'")
+    ap.add_argument("--only", choices=["scf", "poisson"], default=None,
+                    help="profiling aid: run only the SCF or only the Poisson comparison and "
+                         "print its JSON object (not the benchmark line)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one device copy of a 512-vector FP32 block (known bytes) so "
                          "that a profile holds a plain copy kernel to scale HBM counters by")
@@ -283,6 +286,13 @@ def main(argv=None):
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is described in DESIGN.md, not run)"}))
         return 2
+
+    if args.only == "scf":
+        print(json.dumps({name: scf_bench(name) for name in args.scf.split(",")}))
+        return 0
+    if args.only == "poisson":
+        print(json.dumps(poisson_bench()))
+        return 0
 
     from sparc_amd import gpu
     from sparc_amd.workloads import operator_filter_workload
