@@ -112,8 +112,8 @@ struct sgpu_context {
     // radius-6 tile shape (TX x TY, RY rows per thread): 0 = 32x16x2 (default,
     // fastest measured on MI355X), 1 = 32x32x4, 2 = 64x16x4, 3 = 32x32x8,
     // 4 = 64x32x8, 5 / 6 = 32x16x2 / 32x8x2 with an 8-plane LDS ring (FP32;
-    // FP64 uses 0); set from the SPARC_AMD_STENCIL_TILE environment variable
-    // at creation
+    // FP64 uses 0), 7 = 32x32x2, 8 = 64x16x2 (512 threads); set from the
+    // SPARC_AMD_STENCIL_TILE environment variable at creation
     int stencil_cfg = 0;
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;

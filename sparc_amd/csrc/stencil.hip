@@ -169,7 +169,11 @@ __global__ __launch_bounds__(TX *(TY / RY)) void stencil_tiled(StencilParams<T> 
 #pragma unroll
         for (int r = 0; r < RY; r++) q[r][s + 1] = (za >= 0 && coff[r] >= 0) ? xa[coff[r]] : T(0);
     }
-    T nq[RY], nh[HPT];
+    // nv / nz2: Veff and the recurrence's z term of the next output plane,
+    // loaded one step ahead like the x values
+    const bool has_v = p.bdiag != T(0), has_z = p.z != nullptr;
+    const T *zc = has_z ? p.z + col * p.ldz : nullptr;
+    T nq[RY], nh[HPT], nv[RY], nz2[RY];
     {
         const int zn = axis_index(zs + kR, p.nz, p.bcz);
         const T *xn = x + (long)(zn < 0 ? 0 : zn) * nxny;
@@ -178,14 +182,23 @@ __global__ __launch_bounds__(TX *(TY / RY)) void stencil_tiled(StencilParams<T> 
         const T *xz = x + (long)zs * nxny;
 #pragma unroll
         for (int h = 0; h < HPT; h++) nh[h] = hoff[h] >= 0 ? xz[hoff[h]] : T(0);
+        const long zb = (long)zs * nxny;
+#pragma unroll
+        for (int r = 0; r < RY; r++) {
+            nv[r] = (has_v && ooff[r] >= 0) ? p.v[zb + ooff[r]] : T(0);
+            nz2[r] = (has_z && ooff[r] >= 0) ? zc[zb + ooff[r]] : T(0);
+        }
     }
     int buf = 0;
     for (int z = zs; z < ze; z++) {
+        T cv[RY], cz[RY];
 #pragma unroll
         for (int r = 0; r < RY; r++) {
 #pragma unroll
             for (int s = 0; s < 2 * kR; s++) q[r][s] = q[r][s + 1];
             q[r][2 * kR] = nq[r];
+            cv[r] = nv[r];
+            cz[r] = nz2[r];
         }
         T hv[HPT];
 #pragma unroll
@@ -198,6 +211,12 @@ __global__ __launch_bounds__(TX *(TY / RY)) void stencil_tiled(StencilParams<T> 
             const T *xz = x + (long)(z + 1) * nxny;
 #pragma unroll
             for (int h = 0; h < HPT; h++) nh[h] = hoff[h] >= 0 ? xz[hoff[h]] : T(0);
+            const long zb1 = (long)(z + 1) * nxny;
+#pragma unroll
+            for (int r = 0; r < RY; r++) {
+                nv[r] = (has_v && ooff[r] >= 0) ? p.v[zb1 + ooff[r]] : T(0);
+                nz2[r] = (has_z && ooff[r] >= 0) ? zc[zb1 + ooff[r]] : T(0);
+            }
         }
         // double-buffered plane: one barrier per step (a thread can only
         // overwrite this buffer two steps later, after everyone passed the
@@ -225,9 +244,9 @@ __global__ __launch_bounds__(TX *(TY / RY)) void stencil_tiled(StencilParams<T> 
                 res += rx + ry + rz;
             }
             const long o = zb + ooff[r];
-            if (p.bdiag != T(0)) res += p.bdiag * (p.v[o] * xc);
+            if (has_v) res += p.bdiag * (cv[r] * xc);
             T val = p.s1 * res;
-            if (p.z) val += p.s2 * p.z[col * p.ldz + o];
+            if (has_z) val += p.s2 * cz[r];
             p.out[col * p.ldo + o] = val;
         }
     }
@@ -447,6 +466,8 @@ int launch_typed(sgpu_context *ctx, int ncol, double a, double bdiag, double c, 
             case 2: launch_tiled<T, 64, 16, 4>(ctx, p, ncol); break;
             case 3: launch_tiled<T, 32, 32, 8>(ctx, p, ncol); break;
             case 4: launch_tiled<T, 64, 32, 8>(ctx, p, ncol); break;
+            case 7: launch_tiled<T, 32, 32, 2>(ctx, p, ncol); break;
+            case 8: launch_tiled<T, 64, 16, 2>(ctx, p, ncol); break;
             default: launch_tiled<T, 32, 16, 2>(ctx, p, ncol); break;
         }
     } else {
