@@ -58,6 +58,34 @@ int apply_h(sgpu_context *ctx, int dtype, int ncol, double c, const void *x, lon
     return SGPU_OK;
 }
 
+// apply_h for one step of a chain in which each out is the next x (the
+// Chebyshev recurrence): with want_next the nonlocal partials of out are
+// formed inside the nonlocal apply and the next step reuses them instead of
+// re-reading its input (*have_part carries that from step to step).
+int apply_h_chain(sgpu_context *ctx, int dtype, int ncol, double c, const void *x, long ldx,
+                  void *out, long ldo, const void *z, long ldz, double s1, double s2,
+                  bool *have_part, bool want_next)
+{
+    if (!ctx->has_veff) {
+        sgpu_set_error("Hamiltonian apply before sgpu_set_veff");
+        return SGPU_ERR_STATE;
+    }
+    sgpu_tmark t;
+    SGPU_CHECK(sgpu_tbegin(ctx, &t));
+    SGPU_CHECK(sgpu_launch_stencil(ctx, dtype, ncol, -0.5, 1.0, c, veff_for(ctx, dtype), x, ldx, out,
+                                   ldo, z, ldz, s1, s2, true));
+    SGPU_CHECK(sgpu_tend(ctx, &t, "stencil"));
+    bool projected = false;
+    if (ctx->nl.n_infl > 0) {
+        SGPU_CHECK(sgpu_tbegin(ctx, &t));
+        SGPU_CHECK(sgpu_launch_vnl_chain(ctx, dtype, ncol, x, ldx, out, ldo, s1, *have_part,
+                                         want_next && ctx->nl_chain, &projected));
+        SGPU_CHECK(sgpu_tend(ctx, &t, "nonlocal"));
+    }
+    *have_part = projected;
+    return SGPU_OK;
+}
+
 // Chebyshev filter on raw device storage (ChebyshevFiltering,
 // src/eigenSolver.c:722-798).  On return *res points at the buffer holding
 // Y_m and *prev at the one holding Y_{m-1}; bx starts as X, by as Y.
@@ -70,13 +98,16 @@ int filter_raw(sgpu_context *ctx, int dtype, int ncol, void *bx, void *by, long 
     sgpu_tmark tall;
     SGPU_CHECK(sgpu_tbegin(ctx, &tall));
     // Y = (sigma1 / e) (H - c) X
-    SGPU_CHECK(apply_h(ctx, dtype, ncol, -c, bx, ld, by, ld, nullptr, 0, sigma1 / e, 0.0));
+    bool have_part = false;
+    SGPU_CHECK(apply_h_chain(ctx, dtype, ncol, -c, bx, ld, by, ld, nullptr, 0, sigma1 / e, 0.0,
+                             &have_part, m > 1));
     void *P = by, *Q = bx;
     for (int j = 1; j < m; j++) {
         const double sigma2 = 1.0 / (gamma - sigma);
         const double vs = 2.0 * sigma2 / e, vs2 = sigma * sigma2;
         // Q <- vs (H - c) P - vs2 Q   (in place over Y_{j-1})
-        SGPU_CHECK(apply_h(ctx, dtype, ncol, -c, P, ld, Q, ld, Q, ld, vs, -vs2));
+        SGPU_CHECK(apply_h_chain(ctx, dtype, ncol, -c, P, ld, Q, ld, Q, ld, vs, -vs2, &have_part,
+                                 j + 1 < m));
         std::swap(P, Q);
         sigma = sigma2;
     }

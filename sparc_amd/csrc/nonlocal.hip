@@ -238,9 +238,19 @@ __global__ __launch_bounds__(256) void nlb_apply(T *__restrict__ out, long ldo, 
     const int li = lane & 15, lk = lane >> 4;
     for (int pt_tile = wave; pt_tile < kBP / 16; pt_tile += kWaves) {
         const int q0 = pt_tile * 16;
-        typename M::acc acc[NCT];
+        // the tile's current output values are loaded before the MFMA chain
+        // so that their latency overlaps it
+        const long pt = brick_point(q0 + li, x0, y0, z0, nx, ny, nz);
+        typename M::acc acc[NCT], ov[NCT];
 #pragma unroll
-        for (int ct = 0; ct < NCT; ct++) acc[ct] = typename M::acc{0, 0, 0, 0};
+        for (int ct = 0; ct < NCT; ct++) {
+            acc[ct] = typename M::acc{0, 0, 0, 0};
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int col = col0 + ct * 16 + M::drow(lk, r);
+                ov[ct][r] = (pt >= 0 && col < ncol) ? out[(long)col * ldo + pt] : T(0);
+            }
+        }
         // Phi loads of up to 8 j-steps issued before their MFMAs
         for (int jb = 0; jb < np; jb += 32) {
             T bqs[8];
@@ -259,14 +269,13 @@ __global__ __launch_bounds__(256) void nlb_apply(T *__restrict__ out, long ldo, 
                 }
             }
         }
-        const long pt = brick_point(q0 + li, x0, y0, z0, nx, ny, nz);
         if (pt < 0) continue;
 #pragma unroll
         for (int ct = 0; ct < NCT; ct++) {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int col = col0 + ct * 16 + M::drow(lk, r);
-                if (col < ncol) out[(long)col * ldo + pt] += acc[ct][r];
+                if (col < ncol) out[(long)col * ldo + pt] = ov[ct][r] + acc[ct][r];
             }
         }
     }
@@ -320,6 +329,7 @@ __global__ __launch_bounds__(256) void nls_project(const T *__restrict__ X, long
         for (int ct = 0; ct < NCT; ct++) acc[rt][ct] = typename M::acc{0, 0, 0, 0};
     for (int qc = 0; qc < qpad; qc += kQC) {
         const int qn = min(kQC, qpad - qc);
+#pragma unroll 8
         for (int t = threadIdx.x; t < qn * NC; t += 256) {
             const int c = t / qn, q = t - c * qn;
             const int g = gp[qc + q];
@@ -419,9 +429,18 @@ __global__ __launch_bounds__(256) void nls_apply(T *__restrict__ out, long ldo, 
     const int li = lane & 15, lk = lane >> 4;
     for (int qt = wave; qt < qpad / 16; qt += kWaves) {
         const int q0 = qt * 16;
-        typename M::acc acc[NCT];
+        // current output values loaded ahead of the MFMA chain
+        const int g = gpos[poff + q0 + li];
+        typename M::acc acc[NCT], ov[NCT];
 #pragma unroll
-        for (int ct = 0; ct < NCT; ct++) acc[ct] = typename M::acc{0, 0, 0, 0};
+        for (int ct = 0; ct < NCT; ct++) {
+            acc[ct] = typename M::acc{0, 0, 0, 0};
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int col = col0 + ct * 16 + M::drow(lk, r);
+                ov[ct][r] = (g >= 0 && col < ncol) ? out[(long)col * ldo + g] : T(0);
+            }
+        }
         for (int jb = 0; jb < npad; jb += 16) {
             // npad is a multiple of 16: four p-steps, Chi loads first
             T bqs[4];
@@ -435,16 +454,163 @@ __global__ __launch_bounds__(256) void nls_apply(T *__restrict__ out, long ldo, 
                     acc[ct] = M::run(beta[(j + lk) * LDB + ct * 16 + li], bqs[u], acc[ct]);
             }
         }
-        const int g = gpos[poff + q0 + li];
         if (g < 0) continue;
 #pragma unroll
         for (int ct = 0; ct < NCT; ct++) {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int col = col0 + ct * 16 + M::drow(lk, r);
-                if (col < ncol) out[(long)col * ldo + g] += acc[ct][r];
+                if (col < ncol) out[(long)col * ldo + g] = ov[ct][r] + acc[ct][r];
             }
         }
+    }
+}
+
+// Chained applies (the Chebyshev recurrence, where every Hamiltonian output
+// is the next input), sphere layout with one colour (no two entries share a
+// grid point): after an entry's apply its points hold final values, so the
+// entry's partials for the next application are formed here from the tile
+// still in registers instead of re-gathering the block in nls_project.  Each
+// wave's 16-point x NC-column result is transposed through a wave-private
+// LDS slice into the MFMA B layout (points along k); Chi is the A operand
+// from the [p][qpad] copy.  The partials go to the other partials buffer
+// (the atom's images still read this application's partials for beta).
+// Wave w takes, chunk by chunk, the quarter of the points that nls_project
+// gives it, in the same order, and the waves are summed in the same fixed
+// order, so the partials are bitwise those of the unfused path.
+template <typename T>
+__global__ __launch_bounds__(256) void nls_apply_proj(T *__restrict__ out, long ldo, int ncol,
+                                                      int e0, const int *__restrict__ ent_info,
+                                                      const long *__restrict__ ent_chi,
+                                                      const int *__restrict__ gpos,
+                                                      const T *__restrict__ chit,
+                                                      const int *__restrict__ atom_img_ptr,
+                                                      const int *__restrict__ atom_img,
+                                                      const int *__restrict__ atom_row,
+                                                      const double *__restrict__ gamma,
+                                                      double scale_dv, const T *__restrict__ part,
+                                                      int max_np, T *__restrict__ part_next)
+{
+    using M = Mfma<T>;
+    using acc_t = typename M::acc;
+    constexpr int NC = M::NCA, NCT = NC / 16, LDB = NC * 3 / 2, LDY = NC + 1, MRT = 4;
+    __shared__ T beta[64 * LDB];
+    __shared__ T ytile[kWaves * 16 * LDY];
+    __shared__ acc_t red[MRT * NCT][64];
+    const int lid = xcd_remap(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int e = e0 + lid / gridDim.x;
+    const int col0 = (lid % gridDim.x) * NC;
+    const int qpad = ent_info[6 * e], poff = ent_info[6 * e + 1];
+    const int np = ent_info[6 * e + 2], npad = ent_info[6 * e + 3], atom = ent_info[6 * e + 4];
+    const int nrt = npad / 16;
+    const int i0 = atom_img_ptr[atom], i1 = atom_img_ptr[atom + 1], row = atom_row[atom];
+    for (int t = threadIdx.x; t < npad * NC; t += 256) {
+        const int p = t / NC, c = t - p * NC;
+        T v = T(0);
+        if (p < np && col0 + c < ncol) {
+            for (int i = i0; i < i1; i++) v += part[((long)atom_img[i] * max_np + p) * ncol + col0 + c];
+            v *= (T)(scale_dv * gamma[row + p]);
+        }
+        beta[p * LDB + c] = v;
+    }
+    __syncthreads();
+    const T *ct_ = chit + ent_chi[e];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    T *yw = ytile + wave * 16 * LDY;
+    acc_t pacc[MRT][NCT];
+#pragma unroll
+    for (int rt = 0; rt < MRT; rt++)
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++) pacc[rt][ct] = acc_t{0, 0, 0, 0};
+    for (int qc = 0; qc < qpad; qc += kQC) {
+        const int qn = min(kQC, qpad - qc);  // a multiple of 64: same tile count per wave
+        for (int tt = 0; tt < qn / 64; tt++) {
+            const int q0 = qc + wave * (qn / 4) + tt * 16;
+            const int g = gpos[poff + q0 + li];
+            acc_t acc[NCT], ov[NCT];
+#pragma unroll
+            for (int ct = 0; ct < NCT; ct++) {
+                acc[ct] = acc_t{0, 0, 0, 0};
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int col = col0 + ct * 16 + M::drow(lk, r);
+                    ov[ct][r] = (g >= 0 && col < ncol) ? out[(long)col * ldo + g] : T(0);
+                }
+            }
+            for (int jb = 0; jb < npad; jb += 16) {
+                T bqs[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) bqs[u] = ct_[(long)(jb + 4 * u + lk) * qpad + q0 + li];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int j = jb + 4 * u;
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ct++)
+                        acc[ct] = M::run(beta[(j + lk) * LDB + ct * 16 + li], bqs[u], acc[ct]);
+                }
+            }
+#pragma unroll
+            for (int ct = 0; ct < NCT; ct++) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int cl = ct * 16 + M::drow(lk, r);
+                    const bool ok = g >= 0 && col0 + cl < ncol;
+                    const T y = ov[ct][r] + acc[ct][r];
+                    if (ok) out[(long)(col0 + cl) * ldo + g] = y;
+                    yw[li * LDY + cl] = ok ? y : T(0);
+                }
+            }
+            __syncthreads();
+            T av[4][MRT];
+#pragma unroll
+            for (int s4 = 0; s4 < 4; s4++)
+#pragma unroll
+                for (int rt = 0; rt < MRT; rt++)
+                    av[s4][rt] = rt < nrt ? ct_[(long)(rt * 16 + li) * qpad + q0 + 4 * s4 + lk] : T(0);
+#pragma unroll
+            for (int s4 = 0; s4 < 4; s4++) {
+                T bv[NCT];
+#pragma unroll
+                for (int ct = 0; ct < NCT; ct++) bv[ct] = yw[(4 * s4 + lk) * LDY + ct * 16 + li];
+#pragma unroll
+                for (int rt = 0; rt < MRT; rt++) {
+                    if (rt < nrt) {
+#pragma unroll
+                        for (int ct = 0; ct < NCT; ct++)
+                            pacc[rt][ct] = M::run(av[s4][rt], bv[ct], pacc[rt][ct]);
+                    }
+                }
+            }
+        }
+    }
+    // fixed-order cross-wave sum: wave 3 -> 2 -> 1 -> 0
+    for (int w = kWaves - 1; w >= 1; w--) {
+        if (wave == w)
+#pragma unroll
+            for (int rt = 0; rt < MRT; rt++)
+#pragma unroll
+                for (int ct = 0; ct < NCT; ct++) red[rt * NCT + ct][lane] = pacc[rt][ct];
+        __syncthreads();
+        if (wave == w - 1)
+#pragma unroll
+            for (int rt = 0; rt < MRT; rt++)
+#pragma unroll
+                for (int ct = 0; ct < NCT; ct++) pacc[rt][ct] += red[rt * NCT + ct][lane];
+        __syncthreads();
+    }
+    if (wave == 0) {
+#pragma unroll
+        for (int rt = 0; rt < MRT; rt++)
+#pragma unroll
+            for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int p = rt * 16 + M::drow(lk, r);
+                    const int col = col0 + ct * 16 + li;
+                    if (p < np && col < ncol)
+                        part_next[((long)e * max_np + p) * ncol + col] = pacc[rt][ct][r];
+                }
     }
 }
 
@@ -510,6 +676,72 @@ int launch_vnl_typed(sgpu_context *ctx, int ncol, const T *x, long ldx, T *out, 
         out, ldo, ncol, ctx->nx, ctx->ny, ctx->nz, nl.brick_org, nl.brick_np, nl.brick_row,
         nl.row_alpha, phic, alpha_s);
     SGPU_HIP(hipGetLastError());
+    SGPU_CHECK(sgpu_tend(ctx, &t, "nl_apply"));
+    return SGPU_OK;
+}
+
+int chain_buffer(sgpu_context *ctx, size_t bytes, void **p)
+{
+    sgpu_nonlocal &nl = ctx->nl;
+    if (bytes > nl.chain_bytes) {
+        SGPU_HIP(hipStreamSynchronize(ctx->stream));
+        if (nl.chain) SGPU_HIP(hipFree(nl.chain));
+        nl.chain = nullptr;
+        nl.chain_bytes = 0;
+        SGPU_HIP(hipMalloc(&nl.chain, bytes));
+        nl.chain_bytes = bytes;
+    }
+    *p = nl.chain;
+    return SGPU_OK;
+}
+
+template <typename T>
+int launch_vnl_chain_sphere(sgpu_context *ctx, int ncol, const T *x, long ldx, T *out, long ldo,
+                            double scale, const T *chir, const T *chit, bool have_part,
+                            bool want_next, bool *projected)
+{
+    sgpu_nonlocal &nl = ctx->nl;
+    *projected = false;
+    if (nl.n_ent == 0 || ncol <= 0) return SGPU_OK;
+    constexpr int NCP = Mfma<T>::NCP, NC = Mfma<T>::NCA;
+    const int max_np = 64;
+    const size_t part_elems = (size_t)nl.n_ent * max_np * ncol;
+    const size_t need = 2 * part_elems * sizeof(T) + 256;
+    if (have_part && need > nl.chain_bytes) {
+        sgpu_set_error("nonlocal chain: partials of a different shape");
+        return SGPU_ERR_STATE;
+    }
+    void *buf = nullptr;
+    SGPU_CHECK(chain_buffer(ctx, need, &buf));
+    if (!have_part) nl.chain_cur = 0;
+    T *part = (T *)buf + (size_t)nl.chain_cur * part_elems;
+    T *part_next = (T *)buf + (size_t)(nl.chain_cur ^ 1) * part_elems;
+    sgpu_tmark t;
+    if (!have_part) {
+        SGPU_CHECK(sgpu_tbegin(ctx, &t));
+        nls_project<T><<<dim3((ncol + NCP - 1) / NCP, nl.n_ent), 256, 0, ctx->stream>>>(
+            x, ldx, ncol, nl.ent_info, nl.ent_chi, nl.gpos, chir, max_np, part);
+        SGPU_HIP(hipGetLastError());
+        SGPU_CHECK(sgpu_tend(ctx, &t, "nl_project"));
+    }
+    SGPU_CHECK(sgpu_tbegin(ctx, &t));
+    if (want_next && nl.n_colors == 1) {
+        nls_apply_proj<T><<<dim3((ncol + NC - 1) / NC, nl.n_ent), 256, 0, ctx->stream>>>(
+            out, ldo, ncol, 0, nl.ent_info, nl.ent_chi, nl.gpos, chit, nl.atom_img_ptr,
+            nl.atom_img, nl.atom_row, nl.gamma, scale * nl.dV, part, max_np, part_next);
+        SGPU_HIP(hipGetLastError());
+        nl.chain_cur ^= 1;
+        *projected = true;
+    } else {
+        for (int c = 0; c < nl.n_colors; c++) {
+            const int e0 = nl.color_start[c], ne = nl.color_start[c + 1] - e0;
+            if (ne <= 0) continue;
+            nls_apply<T><<<dim3((ncol + NC - 1) / NC, ne), 256, 0, ctx->stream>>>(
+                out, ldo, ncol, e0, nl.ent_info, nl.ent_chi, nl.gpos, chit, nl.atom_img_ptr,
+                nl.atom_img, nl.atom_row, nl.gamma, scale * nl.dV, part, max_np);
+            SGPU_HIP(hipGetLastError());
+        }
+    }
     SGPU_CHECK(sgpu_tend(ctx, &t, "nl_apply"));
     return SGPU_OK;
 }
@@ -636,7 +868,7 @@ void sgpu_free_nonlocal(sgpu_nonlocal &nl)
                     nl.phi32,     nl.phic64,   nl.phic32,     nl.arow_ptr,  nl.arow_src,
                     nl.gamma,     nl.ent_info, nl.ent_chi,    nl.gpos,      nl.chir64,
                     nl.chir32,    nl.chit64,   nl.chit32,     nl.atom_img_ptr, nl.atom_img,
-                    nl.atom_row};
+                    nl.atom_row,  nl.chain};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     nl = sgpu_nonlocal();
@@ -831,4 +1063,33 @@ int sgpu_launch_vnl(sgpu_context *ctx, int dtype, int ncol, const void *x, long 
                                         scale, ctx->nl.phi64, ctx->nl.phic64);
     return launch_vnl_typed<float>(ctx, ncol, (const float *)x, ldx, (float *)out, ldo, scale,
                                    ctx->nl.phi32, ctx->nl.phic32);
+}
+
+int sgpu_launch_vnl_chain(sgpu_context *ctx, int dtype, int ncol, const void *x, long ldx,
+                          void *out, long ldo, double scale, bool have_part, bool want_next,
+                          bool *projected)
+{
+    sgpu_nonlocal &nl = ctx->nl;
+    *projected = false;
+    if (have_part && nl.chain_x != x) {
+        sgpu_set_error("nonlocal chain: no partials held for this block");
+        return SGPU_ERR_STATE;
+    }
+    nl.chain_x = nullptr;
+    // brick layout: a fused apply + projection kept the projection's
+    // accumulators for up to 128 projector rows live across the brick and ran
+    // at 2 waves per SIMD, slower (3.25 ms per application on the 96^3 x 512
+    // operator case) than the two kernels (2.55 ms); only the sphere layout
+    // carries partials
+    if (nl.mode != 1) return sgpu_launch_vnl(ctx, dtype, ncol, x, ldx, out, ldo, scale);
+    const int st =
+        dtype == SGPU_F64
+            ? launch_vnl_chain_sphere<double>(ctx, ncol, (const double *)x, ldx, (double *)out, ldo,
+                                              scale, nl.chir64, nl.chit64, have_part, want_next,
+                                              projected)
+            : launch_vnl_chain_sphere<float>(ctx, ncol, (const float *)x, ldx, (float *)out, ldo,
+                                             scale, nl.chir32, nl.chit32, have_part, want_next,
+                                             projected);
+    if (st == SGPU_OK && *projected) nl.chain_x = out;
+    return st;
 }

@@ -93,6 +93,12 @@ struct sgpu_nonlocal {
     int *atom_img_ptr = nullptr;  // atom a: entries atom_img[ptr[a]..ptr[a+1])
     int *atom_img = nullptr;
     int *atom_row = nullptr;      // alpha row of the atom's first projector
+    // partials carried from one application of a chain to the next: two
+    // sphere partial buffers (chain_cur: the current one)
+    void *chain = nullptr;
+    size_t chain_bytes = 0;
+    int chain_cur = 0;
+    const void *chain_x = nullptr;  // the block whose partials are held
 };
 
 struct sgpu_timer_acc {
@@ -115,6 +121,9 @@ struct sgpu_context {
     // FP64 uses 0), 7 = 32x32x2, 8 = 64x16x2 (512 threads); set from the
     // SPARC_AMD_STENCIL_TILE environment variable at creation
     int stencil_cfg = 0;
+    // fuse each filter step's nonlocal projection into the previous step's
+    // apply (SPARC_AMD_NL_CHAIN=0 turns it off; results are bitwise equal)
+    bool nl_chain = true;
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;
 
@@ -181,6 +190,15 @@ void sgpu_free_nonlocal(sgpu_nonlocal &nl);
 // out += scale * Vnl x  (alpha computed from x); dtype of x/out.
 int sgpu_launch_vnl(sgpu_context *ctx, int dtype, int ncol, const void *x, long ldx, void *out,
                     long ldo, double scale);
+// The same for one application of a chain in which each out is the next x
+// (the Chebyshev recurrence).  have_part: the partials of x were left by the
+// previous call (its *projected was true, and x is that call's out,
+// unchanged since); want_next: also form the partials of out after the
+// apply, fused into it, where the layout allows (the sphere layout with one
+// colour; *projected tells).
+int sgpu_launch_vnl_chain(sgpu_context *ctx, int dtype, int ncol, const void *x, long ldx,
+                          void *out, long ldo, double scale, bool have_part, bool want_next,
+                          bool *projected);
 // ---- aar.hip ----
 void anderson_coeff(int m, const double *FtF, const double *Ftf, double *G);
 int sgpu_anderson_coeff_dev(sgpu_context *ctx, const double *F, long ldf, int m, const double *f,

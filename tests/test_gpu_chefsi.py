@@ -185,3 +185,52 @@ def test_fp32_gram_accuracy(gpu, ns):
         Hp, Mp, _ = ctx.DP_Project_Hamiltonian(Y, gpu.F32)
     ref = Y32.T @ Y32
     assert np.max(np.abs(Mp - ref)) < 5e-6 * np.max(np.abs(ref))
+
+
+def _separate_sites_case(seed):
+    """Four projector sites whose 1.6-Bohr spheres do not overlap (one
+    colour in the sphere layout), one of them across the cell face and split
+    into two image entries that share alpha rows."""
+    from sparc_amd.workloads import Grid, gaussian_projectors, gaussian_veff
+    from tests.helpers import split_images
+    g = Grid(20, 20, 20, 0.5, 0.5, 0.5)
+    rng = np.random.default_rng(seed)
+    veff = gaussian_veff(g, 4, rng)
+    sites = np.array([[2.5, 2.5, 2.5], [7.5, 7.5, 2.5], [2.5, 7.5, 7.5], [0.2, 5.0, 7.0]])
+    nl = gaussian_projectors(g, sites, 1.6, sigmas=(1.0, 0.7), gammas=(1.3, -0.6), lmax=1)
+    return g, veff, split_images(nl, 3), rng
+
+
+@pytest.mark.parametrize("dtype", [0, 1])
+@pytest.mark.parametrize("overlap", [False, True])
+@pytest.mark.parametrize("mode", ["brick", "sphere"])
+def test_filter_nonlocal_chain(gpu, oracle_mod, monkeypatch, mode, overlap, dtype):
+    """The filter forms each step's nonlocal partials inside the previous
+    step's apply.  Against the unfused path (SPARC_AMD_NL_CHAIN=0): bitwise
+    equal; against the oracle: the usual tolerance.  Both layouts; separate
+    spheres (one colour: the sphere layout fuses) and overlapping ones (the
+    sphere layout applies colour by colour, unfused)."""
+    from tests.helpers import split_images
+    monkeypatch.setenv("SPARC_AMD_NL_MODE", mode)
+    if overlap:
+        g, veff, nl, rng = small_case(n=(16, 15, 14), n_sites=5, rc=2.6, seed=24, lmax=1)
+        nl = split_images(nl, 1)
+    else:
+        g, veff, nl, rng = _separate_sites_case(25)
+    o = oracle_mod.Oracle(g, veff, nl)
+    a0, b, a = _bounds(o, g)
+    ncol = 37
+    X = rng.uniform(-0.5, 0.5, (g.nd, ncol))
+    Yo, _ = o.chebyshev_filter(X, 9, a, b, a0)
+    out = {}
+    for chain in ("1", "0"):
+        monkeypatch.setenv("SPARC_AMD_NL_CHAIN", chain)
+        with gpu.Context(g) as ctx:
+            ctx.set_veff(veff)
+            ctx.set_nonlocal(nl)
+            bx = ctx.block(ncol, dtype).upload(X)
+            by = ctx.block(ncol, dtype)
+            ctx.chebyshev_filter_dev(bx, by, 9, a, b, a0)
+            out[chain] = by.download()
+    assert np.array_equal(out["1"], out["0"])
+    assert rel_err(out["1"], Yo) < (1e-11 if dtype == 0 else 1e-4)
