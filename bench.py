@@ -122,8 +122,10 @@ def cpu_baseline(wl, ncol_sample=48, degree=None):
                       f"vectors, degree {m}, {dt:.2f} s"}
 
 
-def scf_bench(name):
-    """FP64 and mixed-precision device SCF of one psp8 system."""
+def scf_bench(name, unet_weights=None):
+    """FP64 and mixed-precision device SCF of one psp8 system, plus the mixed
+    run with the learned Poisson preconditioner when trained weights are
+    given and the grid allows it (dimensions multiples of 4, periodic)."""
     from sparc_amd import gpu
     from sparc_amd.systems import scf_system
     t0 = time.perf_counter()
@@ -140,13 +142,22 @@ def scf_bench(name):
                        + ("LDA_PW" if p["xc"] == 1 else "LDA_PZ") + ", "
                        + ("Fermi-Dirac" if p["smear_type"] == 0 else "Gaussian")
                        + f" smearing {1.0 / p['beta']:.4g} Ha, tol_scf {p['tol_scf']:.3g}"}
+    learned = unet_weights is not None and all(n % 4 == 0 for n in (g.nx, g.ny, g.nz))
+    plan = [("f64", gpu.F64, False), ("mixed", gpu.F32, False)]
+    if learned:
+        plan.append(("mixed_learned", gpu.F32, True))
     runs = {}
     with gpu.Context(g) as ctx:
         ctx.set_nonlocal(st.nloc)
-        for tag, dt in (("f64", gpu.F64), ("mixed", gpu.F32)):
+        net = None
+        if learned:
+            net = ctx.unet(seed=1)
+            net.set_weights(unet_weights)
+        for tag, dt, use_net in plan:
             ctx.timer_reset()
             ctx.timer_enable(True)
-            r = ctx.scf_run(st.b, st.rho_at, dict(p, dtype=dt), rho_core=st.rho_core)
+            r = ctx.scf_run(st.b, st.rho_at, dict(p, dtype=dt), rho_core=st.rho_core,
+                            poisson_net=net if use_net else None)
             ctx.timer_enable(False)
             phases = {}
             for k in ("scf_lanczos", "scf_chefsi", "scf_poisson", "scf_kerker",
@@ -160,29 +171,36 @@ def scf_bench(name):
                         "total_ms": round(r["total_ms"], 1), "scf_iters": r["scf_iters"],
                         "converged": bool(r["converged"]), "Etot": r["Etot"],
                         "Etot_per_atom": r["Etot"] / st.n_atom,
+                        "poisson_iters": int(r["poisson_iters"]),
+                        "poisson_fallbacks": int(r["poisson_fallbacks"]),
+                        "kerker_iters": int(r["kerker_iters"]),
                         "phase_ms_per_iter": phases}
             print(f"[bench] scf {name} {tag}: {r['scf_iters']} iters, "
-                  f"{r['mean_iter_ms']:.2f} ms/iter, E/atom {r['Etot'] / st.n_atom:.10f}",
-                  flush=True)
-    f, m = runs["f64"], runs["mixed"]
-    out["dE_per_atom_mixed_vs_f64"] = abs(m["Etot"] - f["Etot"]) / st.n_atom
-    out["max_dlambda_mixed_vs_f64"] = float(np.max(np.abs(m["lambda_"] - f["lambda_"])))
-    out["speedup_mixed_vs_f64"] = round(f["mean_iter_ms"] / m["mean_iter_ms"], 3)
+                  f"{r['mean_iter_ms']:.2f} ms/iter, E/atom {r['Etot'] / st.n_atom:.10f}, "
+                  f"Poisson AAR iterations {r['poisson_iters']}", flush=True)
+        if net is not None:
+            net.close()
+    f = runs["f64"]
+    for tag in runs:
+        if tag == "f64":
+            continue
+        m = runs[tag]
+        sfx = "" if tag == "mixed" else "_" + tag.split("_", 1)[1]
+        out[f"dE_per_atom_mixed{sfx}_vs_f64"] = abs(m["Etot"] - f["Etot"]) / st.n_atom
+        out[f"max_dlambda_mixed{sfx}_vs_f64"] = float(np.max(np.abs(m["lambda_"] - f["lambda_"])))
+        out[f"speedup_mixed{sfx}_vs_f64"] = round(f["mean_iter_ms"] / m["mean_iter_ms"], 3)
     return out
 
 
-def poisson_bench(train_steps=400):
-    """AAR Poisson solve on the 128^3 / 216-charge case with the Jacobi
-    preconditioner and with the learned U-Net preconditioner, the network
-    trained here on the GPU on pairs harvested from the 64-atom Si SCF's
-    Poisson solves (52^3 grid; the 8-atom cell's 26^3 grid is not a multiple
-    of 4, which the U-Net's two halvings need)."""
+def train_unet(train_steps=400):
+    """Train the learned Poisson preconditioner on the GPU, on (residual,
+    error) pairs harvested from Jacobi-AAR solves of the 64-atom Si SCF's
+    Poisson problems (52^3 grid; the 8-atom cell's 26^3 grid is not a
+    multiple of 4, which the U-Net's two halvings need).  Returns the FP32
+    weights and a record of the training."""
     from sparc_amd import gpu
     from sparc_amd.systems import scf_system
-    from sparc_amd.workloads import poisson_workload
-    out = {}
     st, p = scf_system("si64")
-    t0 = time.perf_counter()
     with gpu.Context(st.grid) as ctx:
         ctx.set_nonlocal(st.nloc)
         rhos = [st.rho_at]
@@ -199,15 +217,23 @@ def poisson_bench(train_steps=400):
         hist = net.train(train_steps, batch=4, lr=2e-3, seed=5)
         tt = time.perf_counter() - tt
         w = net.get_weights()
-        out["training"] = {"pairs": net.npairs, "steps": train_steps, "batch": 4,
-                           "harvest_s": round(th, 2), "train_s": round(tt, 2),
-                           "loss_first": float(hist[:5].mean()), "loss_last": float(hist[-20:].mean()),
-                           "source": "Jacobi-AAR Poisson solves (tol 1e-8) of the si64 SCF's "
-                                     "densities (guess and after 1, 2, 4 SCF iterations), 52^3"}
+        info = {"pairs": net.npairs, "steps": train_steps, "batch": 4,
+                "harvest_s": round(th, 2), "train_s": round(tt, 2),
+                "loss_first": float(hist[:5].mean()), "loss_last": float(hist[-20:].mean()),
+                "source": "Jacobi-AAR Poisson solves (tol 1e-8) of the si64 SCF's densities "
+                          "(guess and after 1, 2, 4 SCF iterations), 52^3"}
         net.close()
-    print(f"[bench] unet: {out['training']['pairs']} pairs, harvest {th:.1f}s, train {tt:.1f}s, "
-          f"loss {out['training']['loss_first']:.3e} -> {out['training']['loss_last']:.3e}",
-          flush=True)
+    print(f"[bench] unet: {info['pairs']} pairs, harvest {th:.1f}s, train {tt:.1f}s, "
+          f"loss {info['loss_first']:.3e} -> {info['loss_last']:.3e}", flush=True)
+    return w, info
+
+
+def poisson_bench(w, training):
+    """AAR Poisson solve on the 128^3 / 216-charge case with the Jacobi
+    preconditioner and with the learned U-Net preconditioner (weights w)."""
+    from sparc_amd import gpu
+    from sparc_amd.workloads import poisson_workload
+    out = {"training": training}
     g, rhs = poisson_workload()
     out["workload"] = ("AAR Poisson, 128^3 periodic h=0.35, 216 Gaussian charges q=4 widths "
                        "0.8-1.5 neutralised, rel tol 1e-8, AAR 0.6/0.6/7/6, x0 = 0")
@@ -271,7 +297,8 @@ def main(argv=None):
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-poisson", action="store_true",
-                    help="skip the Jacobi vs learned-preconditioner Poisson comparison")
+                    help="skip the learned Poisson preconditioner: its training, the 128^3 "
+                         "Jacobi vs learned comparison and the learned SCF run")
     ap.add_argument("--scf", default="si8,si64,sial216",
                     help="comma-separated SCF workloads to time (sic8, si8, si64, sial216) or "
                          "'none'")
@@ -283,15 +310,16 @@ def main(argv=None):
                          "that a profile holds a plain copy kernel to scale HBM counters by")
     args = ap.parse_args(argv)
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not run)"}))
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (the path "
+                                   "would shard by bands across GPUs, which is not built)"}))
         return 2
 
     if args.only == "scf":
-        print(json.dumps({name: scf_bench(name) for name in args.scf.split(",")}))
+        w = None if args.no_poisson else train_unet()[0]
+        print(json.dumps({name: scf_bench(name, w) for name in args.scf.split(",")}))
         return 0
     if args.only == "poisson":
-        print(json.dumps(poisson_bench()))
+        print(json.dumps(poisson_bench(*train_unet())))
         return 0
 
     from sparc_amd import gpu
@@ -364,10 +392,12 @@ def main(argv=None):
         res["speedup_f32_vs_f64"] = round(ms64 / ms32, 3)
         print(f"[bench] fp64 filter {ms64:.2f} ms ({gbs64:.0f} GB/s)", flush=True)
     ctx.close()
+    trained = None if args.no_poisson else train_unet()
     if args.scf and args.scf != "none":
-        res["scf"] = {name: scf_bench(name) for name in args.scf.split(",")}
-    if not args.no_poisson:
-        res["poisson_learned_precond"] = poisson_bench()
+        w = trained[0] if trained else None
+        res["scf"] = {name: scf_bench(name, w) for name in args.scf.split(",")}
+    if trained:
+        res["poisson_learned_precond"] = poisson_bench(*trained)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", flush=True)
         res["cpu_baseline"] = cpu_baseline(wl)

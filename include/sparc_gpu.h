@@ -209,6 +209,11 @@ typedef struct {
     int dtype;              /* SGPU_F64 or SGPU_F32 vector blocks (filter, Gram, rotation) */
     unsigned long long psi_seed;      /* initial orbitals uniform [-0.5, 0.5) */
     unsigned long long lanczos_seed;  /* Lanczos start vector uniform [0, 1) */
+    /* precond_fun of the Poisson AAR solves; NULL selects the reference's
+     * Jacobi_preconditioner.  sgpu_unet_precond with a trained network
+     * (poisson_precond_user = the sgpu_unet) gives the learned solve. */
+    sgpu_precond_fn poisson_precond;
+    void *poisson_precond_user;
 } sgpu_scf_params;
 
 typedef struct {
@@ -219,6 +224,8 @@ typedef struct {
     double mean_iter_ms;    /* mean wall time of iterations 2..N */
     double total_ms;
     long poisson_iters, kerker_iters;  /* AAR iterations summed over the run */
+    int poisson_fallbacks;  /* Poisson solves redone with Jacobi-AAR because the solve with
+                               poisson_precond did not converge */
 } sgpu_scf_result;
 
 /* Density-mixing SCF with CheFSI: replaces scf / scf_loop

@@ -271,7 +271,7 @@ extern "C" int sgpu_scf_run(sgpu_context *ctx, const sgpu_scf_params *par, const
 
     // FP64 device vectors of stride ld, then the Pulay history and small arrays
     enum { RHO_IN, RHO_OUT, XKM1, FK, FKM1, PF, XW, FW, PHI, VXC, EXC, BCH, RHS, LF, ONES, CORE,
-           NVEC };
+           PHI0, NVEC };
     double *mem = nullptr;
     const size_t elems = (size_t)(NVEC + 2 * m) * ld + ns + 64;
     SGPU_HIP(hipMalloc((void **)&mem, elems * sizeof(double)));
@@ -295,9 +295,22 @@ extern "C" int sgpu_scf_run(sgpu_context *ctx, const sgpu_scf_params *par, const
         double prr = 0.0;
         sgpu_tmark tp;
         SGPU_CHECK(sgpu_tbegin(ctx, &tp));
-        int st = sgpu_aar_device(ctx, nullptr, nullptr, nullptr, 0.0, (int)nd, phi, vec[RHS], 0.6,
-                                 0.6, 7, 6, par->tol_poisson, par->maxit_poisson, &pit, &prr);
+        if (par->poisson_precond)
+            SGPU_HIP(hipMemcpyAsync(vec[PHI0], phi, nd * sizeof(double), hipMemcpyDeviceToDevice, s));
+        int st = sgpu_aar_device(ctx, nullptr, par->poisson_precond, par->poisson_precond_user,
+                                 0.0, (int)nd, phi, vec[RHS], 0.6, 0.6, 7, 6, par->tol_poisson,
+                                 par->maxit_poisson, &pit, &prr);
         if (st < 0) return st;
+        if (st != SGPU_OK && par->poisson_precond) {
+            // the solve with the caller's preconditioner did not converge:
+            // Jacobi-AAR (the reference's solve) from the same start
+            res->poisson_iters += pit;
+            res->poisson_fallbacks++;
+            SGPU_HIP(hipMemcpyAsync(phi, vec[PHI0], nd * sizeof(double), hipMemcpyDeviceToDevice, s));
+            st = sgpu_aar_device(ctx, nullptr, nullptr, nullptr, 0.0, (int)nd, phi, vec[RHS], 0.6,
+                                 0.6, 7, 6, par->tol_poisson, par->maxit_poisson, &pit, &prr);
+            if (st < 0) return st;
+        }
         SGPU_CHECK(sgpu_tend(ctx, &tp, "scf_poisson"));
         res->poisson_iters += pit;
         double sum = 0.0;

@@ -21,6 +21,7 @@ missing, loading raises.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 import os
 from pathlib import Path
 
@@ -57,7 +58,8 @@ class SCFParams(C.Structure):
                 ("kerker_ktf", C.c_double), ("kerker_thresh", C.c_double),
                 ("tol_precond", C.c_double), ("Esc", C.c_double), ("xc", C.c_int),
                 ("dtype", C.c_int),
-                ("psi_seed", C.c_ulonglong), ("lanczos_seed", C.c_ulonglong)]
+                ("psi_seed", C.c_ulonglong), ("lanczos_seed", C.c_ulonglong),
+                ("poisson_precond", C.c_void_p), ("poisson_precond_user", C.c_void_p)]
 
 
 class SCFResult(C.Structure):
@@ -66,7 +68,7 @@ class SCFResult(C.Structure):
                 ("E2", C.c_double), ("E3", C.c_double), ("Entropy", C.c_double),
                 ("Efermi", C.c_double), ("scf_error", C.c_double), ("mean_iter_ms", C.c_double),
                 ("total_ms", C.c_double), ("poisson_iters", C.c_long),
-                ("kerker_iters", C.c_long)]
+                ("kerker_iters", C.c_long), ("poisson_fallbacks", C.c_int)]
 
 
 RES_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_double, _dp, _dp, _dp)
@@ -225,6 +227,7 @@ class Context:
                                          grid.hy, grid.hz, grid.bcx, grid.bcy, grid.bcz, grid.order))
         self.h = h
         self._keep = []
+        self._nets = weakref.WeakSet()  # U-Nets hold the context's stream: closed first
 
     # -- plumbing
     def _check(self, st: int) -> int:
@@ -234,6 +237,8 @@ class Context:
 
     def close(self):
         if getattr(self, "h", None):
+            for net in list(getattr(self, "_nets", ())):
+                net.close()
             self.lib.sgpu_destroy(self.h)
             self.h = None
 
@@ -395,10 +400,17 @@ class Context:
         return lam
 
     # -- SCF
-    def scf_run(self, b, rho0, params: dict, rho_core=None):
+    def scf_run(self, b, rho0, params: dict, rho_core=None, poisson_net: "UNet | None" = None):
         """Density-mixing SCF (sgpu_scf_run).  Returns a dict with the energies,
-        eigenvalues, final density / potential and per-iteration history."""
+        eigenvalues, final density / potential and per-iteration history.
+        poisson_net: a trained U-Net of this context as the Poisson solves'
+        preconditioner (None: Jacobi, as the reference)."""
         p = SCFParams(**params)
+        if poisson_net is not None:
+            if poisson_net.ctx is not self:
+                raise ValueError("scf_run: the U-Net belongs to another context")
+            p.poisson_precond = C.cast(self.lib.sgpu_unet_precond, C.c_void_p).value
+            p.poisson_precond_user = poisson_net.h.value
         bb = _f64(b).ravel()
         r0 = _f64(rho0).ravel()
         rc = None if rho_core is None else _f64(rho_core).ravel()
@@ -456,6 +468,7 @@ class UNet:
         h = C.c_void_p()
         ctx._check(self.lib.sgpu_unet_create(ctx.h, 16, seed, C.byref(h)))
         self.h = h
+        ctx._nets.add(self)
 
     def close(self):
         if getattr(self, "h", None):

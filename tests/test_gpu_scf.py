@@ -130,3 +130,65 @@ def test_scf_sic_mixed_matches_reference(sic):
     assert r["converged"] == 1
     assert de < 1e-5
     assert dl < 1e-5
+
+
+@pytest.fixture(scope="module")
+def si8u():
+    """8-atom Si on 28^3 (a grid the U-Net's two halvings accept), its FP64
+    Jacobi SCF and a U-Net trained on the device on (residual, error) pairs
+    of Jacobi-AAR solves of its Poisson problems."""
+    from sparc_amd import gpu
+    from sparc_amd.systems import scf_system
+    gpu.load()
+    st, p = scf_system("si8u")
+    nd = st.grid.nd
+    with gpu.Context(st.grid) as ctx:
+        ctx.set_nonlocal(st.nloc)
+        rj = ctx.scf_run(st.b, st.rho_at, dict(p, dtype=gpu.F64), rho_core=st.rho_core)
+        net = ctx.unet(seed=1)
+        for rho in (st.rho_at, rj["rho"]):
+            net.harvest(4.0 * np.pi * (st.b + rho), np.zeros(nd), tol=1e-8, max_iter=3000,
+                        stride=2, max_pairs=100)
+        hist = net.train(400, batch=4, lr=2e-3, seed=5)
+        w = net.get_weights()
+        net.close()
+    return st, p, rj, w, hist
+
+
+def _learned_scf(st, p, w):
+    from sparc_amd import gpu
+    with gpu.Context(st.grid) as ctx:
+        ctx.set_nonlocal(st.nloc)
+        net = ctx.unet(seed=1)
+        net.set_weights(w)
+        return ctx.scf_run(st.b, st.rho_at, p, rho_core=st.rho_core, poisson_net=net)
+
+
+def test_scf_learned_poisson_preconditioner(si8u):
+    """The SCF with the trained U-Net as precond_fun of its Poisson solves
+    converges to the Jacobi SCF's energy (the solves stop at the same relative
+    residual; tolerance 1e-6 Ha/atom stated here) with fewer Poisson AAR
+    iterations in total and no solve handed back to Jacobi."""
+    from sparc_amd import gpu
+    st, p, rj, w, hist = si8u
+    assert hist[-20:].mean() < 0.5 * hist[:5].mean()
+    ru = _learned_scf(st, dict(p, dtype=gpu.F64), w)
+    assert rj["converged"] and ru["converged"]
+    assert ru["poisson_fallbacks"] == 0
+    assert abs(ru["Etot"] - rj["Etot"]) / st.n_atom < 1e-6
+    assert np.max(np.abs(ru["lambda_"] - rj["lambda_"])) < 1e-5
+    assert ru["poisson_iters"] < rj["poisson_iters"], (ru["poisson_iters"], rj["poisson_iters"])
+
+
+def test_scf_learned_preconditioner_falls_back_to_jacobi(si8u):
+    """A preconditioner that makes AAR diverge (random weights of a large
+    scale) does not derail the SCF: every solve that does not converge within
+    maxit_poisson is redone with Jacobi-AAR from the same start, and the SCF
+    reaches the Jacobi SCF's energy."""
+    from sparc_amd import gpu
+    st, p, rj, w, _ = si8u
+    bad = np.random.default_rng(3).standard_normal(w.size).astype(np.float32) * 0.5
+    ru = _learned_scf(st, dict(p, dtype=gpu.F64, maxit_poisson=300), bad)
+    assert ru["converged"]
+    assert ru["poisson_fallbacks"] > 0
+    assert abs(ru["Etot"] - rj["Etot"]) / st.n_atom < 1e-6
