@@ -379,7 +379,7 @@ def main(argv=None):
         "roofline": {"bound": "hbm", "achieved": round(st_gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(st_gbs / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic() if (args.n, ns, m) == (96, 512, 24) else None,
-                     "kernel": "stencil_tiled<float,32,16,2>",
+                     "kernel": "stencil_tiled<float,%d,%d,%d>" % ctx.stencil_tile(gpu.F32, ns),
                      "per_launch_algorithmic_bytes": stencil_bytes(nd, ns, m, 4) / m},
         "finite": r32["finite"],
     }

@@ -62,6 +62,10 @@ const char *sgpu_last_error(void);
 int sgpu_get_stencil_coeffs(const sgpu_context *ctx, double *d2x, double *d2y, double *d2z);
 /* Synchronise the context's stream. */
 int sgpu_sync(sgpu_context *ctx);
+/* Diagnostic, no counterpart in the reference: the x-y tile (tx x ty points,
+ * ry rows per thread) the radius-6 stencil uses for blocks of ncol columns of
+ * precision dtype; 0 with *tx = 0 if no launch of that shape has chosen one. */
+int sgpu_stencil_tile(const sgpu_context *ctx, int dtype, int ncol, int *tx, int *ty, int *ry);
 
 /* Veff_loc_dmcomm (one spin channel, Nd values).  Replaces the field read by
  * Hamiltonian_vectors_mult (src/hamiltonianVecRoutines.c:45-49). */

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <map>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -115,12 +116,14 @@ struct sgpu_context {
     int bcx = 0, bcy = 0, bcz = 0;
     int fdn = 0;
     double d2x[SGPU_MAX_FDN + 1] = {0}, d2y[SGPU_MAX_FDN + 1] = {0}, d2z[SGPU_MAX_FDN + 1] = {0};
-    // radius-6 tile shape (TX x TY, RY rows per thread): 0 = 32x16x2 (default,
-    // fastest measured on MI355X), 1 = 32x32x4, 2 = 64x16x4, 3 = 32x32x8,
-    // 4 = 64x32x8, 5 / 6 = 32x16x2 / 32x8x2 with an 8-plane LDS ring (FP32;
-    // FP64 uses 0), 7 = 32x32x2, 8 = 64x16x2 (512 threads); set from the
-    // SPARC_AMD_STENCIL_TILE environment variable at creation
-    int stencil_cfg = 0;
+    // radius-6 tile shape (TX x TY, RY rows per thread): 0 = 32x16x2,
+    // 1 = 32x32x4, 2 = 64x16x4, 3 = 32x32x8, 4 = 64x32x8, 5 / 6 = 32x16x2 /
+    // 32x8x2 with an 8-plane LDS ring (FP32; FP64 uses 0), 7 = 32x32x2,
+    // 8 = 64x16x2 (512 threads); -1 (default) = the fastest of 0 / 7 / 8,
+    // timed once per (precision, column count) and kept in tile_choice.
+    // SPARC_AMD_STENCIL_TILE fixes one at creation.
+    int stencil_cfg = -1;
+    std::map<std::pair<int, int>, int> tile_choice;
     // fuse each filter step's nonlocal projection into the previous step's
     // apply (SPARC_AMD_NL_CHAIN=0 turns it off; results are bitwise equal)
     bool nl_chain = true;

@@ -425,6 +425,70 @@ void launch_tiled(sgpu_context *ctx, const StencilParams<T> &p, int ncol)
 }
 
 template <typename T>
+void launch_cfg(sgpu_context *ctx, const StencilParams<T> &p, int ncol, int cfg)
+{
+    switch (cfg) {
+        case 5:
+            if (sizeof(T) == 4) launch_ring<T, 32, 16, 2, 8>(ctx, p, ncol);
+            else launch_tiled<T, 32, 16, 2>(ctx, p, ncol);
+            break;
+        case 6:
+            if (sizeof(T) == 4) launch_ring<T, 32, 8, 2, 8>(ctx, p, ncol);
+            else launch_tiled<T, 32, 16, 2>(ctx, p, ncol);
+            break;
+        case 1: launch_tiled<T, 32, 32, 4>(ctx, p, ncol); break;
+        case 2: launch_tiled<T, 64, 16, 4>(ctx, p, ncol); break;
+        case 3: launch_tiled<T, 32, 32, 8>(ctx, p, ncol); break;
+        case 4: launch_tiled<T, 64, 32, 8>(ctx, p, ncol); break;
+        case 7: launch_tiled<T, 32, 32, 2>(ctx, p, ncol); break;
+        case 8: launch_tiled<T, 64, 16, 2>(ctx, p, ncol); break;
+        default: launch_tiled<T, 32, 16, 2>(ctx, p, ncol); break;
+    }
+}
+
+// Tile shape for one (precision, column count): the fastest of three
+// candidates, each timed on its second launch into a temporary output (the
+// real out may alias z).  Every shape computes each point with the same
+// operations in the same order, so the choice changes no result bit.
+// Measured on MI355X: 32x32x2 fastest on 96^3 x 512, 64x16x2 on 78^3 x 480
+// FP32, 32x16x2 on 52^3 x 160 FP32 -- no single shape wins.
+template <typename T>
+int tune_tile(sgpu_context *ctx, StencilParams<T> p, int ncol, int *best_cfg)
+{
+    static const int cands[] = {0, 7, 8};
+    SGPU_HIP(hipStreamSynchronize(ctx->stream));
+    T *tmp = nullptr;
+    SGPU_HIP(hipMalloc((void **)&tmp, (size_t)ncol * p.ldo * sizeof(T)));
+    p.out = tmp;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int st = SGPU_OK;
+    float best = 0.f;
+    *best_cfg = 0;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) st = SGPU_ERR_HIP;
+    for (int rep = 0; rep < 2 && st == SGPU_OK; rep++) {
+        for (int c : cands) {
+            float ms = 0.f;
+            if (hipEventRecord(e0, ctx->stream) != hipSuccess) { st = SGPU_ERR_HIP; break; }
+            launch_cfg<T>(ctx, p, ncol, c);
+            if (hipGetLastError() != hipSuccess || hipEventRecord(e1, ctx->stream) != hipSuccess ||
+                hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                st = SGPU_ERR_HIP;
+                break;
+            }
+            if (rep == 1 && (c == cands[0] || ms < best)) {
+                best = ms;
+                *best_cfg = c;
+            }
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(tmp);
+    if (st != SGPU_OK) sgpu_set_error("stencil: tile tuning failed");
+    return st;
+}
+
+template <typename T>
 int launch_typed(sgpu_context *ctx, int ncol, double a, double bdiag, double c, const T *v,
                  const T *x, long ldx, T *out, long ldo, const T *z, long ldz, double s1,
                  double s2, bool allow_tiled)
@@ -453,23 +517,18 @@ int launch_typed(sgpu_context *ctx, int ncol, double a, double bdiag, double c, 
         return SGPU_ERR_ARG;
     }
     if (allow_tiled && ctx->fdn == kR) {
-        switch (ctx->stencil_cfg) {
-            case 5:
-                if (sizeof(T) == 4) launch_ring<T, 32, 16, 2, 8>(ctx, p, ncol);
-                else launch_tiled<T, 32, 16, 2>(ctx, p, ncol);
-                break;
-            case 6:
-                if (sizeof(T) == 4) launch_ring<T, 32, 8, 2, 8>(ctx, p, ncol);
-                else launch_tiled<T, 32, 16, 2>(ctx, p, ncol);
-                break;
-            case 1: launch_tiled<T, 32, 32, 4>(ctx, p, ncol); break;
-            case 2: launch_tiled<T, 64, 16, 4>(ctx, p, ncol); break;
-            case 3: launch_tiled<T, 32, 32, 8>(ctx, p, ncol); break;
-            case 4: launch_tiled<T, 64, 32, 8>(ctx, p, ncol); break;
-            case 7: launch_tiled<T, 32, 32, 2>(ctx, p, ncol); break;
-            case 8: launch_tiled<T, 64, 16, 2>(ctx, p, ncol); break;
-            default: launch_tiled<T, 32, 16, 2>(ctx, p, ncol); break;
+        int cfg = ctx->stencil_cfg;
+        if (cfg < 0) {
+            const auto key = std::make_pair((int)sizeof(T), ncol);
+            auto it = ctx->tile_choice.find(key);
+            if (it == ctx->tile_choice.end()) {
+                SGPU_CHECK(tune_tile<T>(ctx, p, ncol, &cfg));
+                ctx->tile_choice[key] = cfg;
+            } else {
+                cfg = it->second;
+            }
         }
+        launch_cfg<T>(ctx, p, ncol, cfg);
     } else {
         dim3 grid((unsigned)((ctx->nd + 255) / 256), ncol);
         switch (ctx->fdn) {
@@ -499,4 +558,29 @@ int sgpu_launch_stencil(sgpu_context *ctx, int dtype, int ncol, double a, double
                                     allow_tiled);
     return launch_typed<float>(ctx, ncol, a, bdiag, c, (const float *)v, (const float *)x, ldx,
                                (float *)out, ldo, (const float *)z, ldz, s1, s2, allow_tiled);
+}
+
+extern "C" int sgpu_stencil_tile(const sgpu_context *ctx, int dtype, int ncol, int *tx, int *ty,
+                                 int *ry)
+{
+    if (!ctx || !tx || !ty || !ry) {
+        sgpu_set_error("sgpu_stencil_tile: bad arguments");
+        return SGPU_ERR_ARG;
+    }
+    int cfg = ctx->stencil_cfg;
+    if (cfg < 0) {
+        auto it = ctx->tile_choice.find(std::make_pair(dtype == SGPU_F64 ? 8 : 4, ncol));
+        if (it == ctx->tile_choice.end()) {
+            *tx = *ty = *ry = 0;
+            return SGPU_OK;
+        }
+        cfg = it->second;
+    }
+    static const int shape[9][3] = {{32, 16, 2}, {32, 32, 4}, {64, 16, 4}, {32, 32, 8}, {64, 32, 8},
+                                    {32, 16, 2}, {32, 8, 2},  {32, 32, 2}, {64, 16, 2}};
+    if (cfg > 8 || ((cfg == 5 || cfg == 6) && dtype == SGPU_F64)) cfg = 0;
+    *tx = shape[cfg][0];
+    *ty = shape[cfg][1];
+    *ry = shape[cfg][2];
+    return SGPU_OK;
 }

@@ -80,6 +80,7 @@ _SIGS = {
     "sgpu_destroy": (C.c_int, [_vp]),
     "sgpu_last_error": (C.c_char_p, []),
     "sgpu_get_stencil_coeffs": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "sgpu_stencil_tile": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _ip, _ip]),
     "sgpu_sync": (C.c_int, [_vp]),
     "sgpu_set_veff": (C.c_int, [_vp, _dp]),
     "sgpu_set_nonlocal": (C.c_int, [_vp, C.c_int, _ip, _ip, _lp, _ip, _lp, _dp, _ip, C.c_int, _dp,
@@ -262,6 +263,13 @@ class Context:
         a, b, c = np.zeros(n), np.zeros(n), np.zeros(n)
         self._check(self.lib.sgpu_get_stencil_coeffs(self.h, _ptr(a), _ptr(b), _ptr(c)))
         return a, b, c
+
+    def stencil_tile(self, dtype: int, ncol: int):
+        """(tx, ty, ry) of the stencil tile used for ncol columns of dtype, or
+        None before a launch of that shape has chosen one."""
+        t = [C.c_int() for _ in range(3)]
+        self._check(self.lib.sgpu_stencil_tile(self.h, dtype, ncol, *[C.byref(v) for v in t]))
+        return None if t[0].value == 0 else tuple(v.value for v in t)
 
     def set_veff(self, veff: np.ndarray):
         v = _f64(veff).ravel()

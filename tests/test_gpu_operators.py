@@ -49,6 +49,35 @@ def test_stencil_tile_shapes(gpu, oracle_mod, monkeypatch, cfg, n, bc):
     assert rel_err(y, o.lap_plus_diag(x, -0.5, 1.0, 0.4, veff)) < 1e-13
 
 
+@pytest.mark.parametrize("dtype", [0, 1])
+def test_stencil_tile_tuning_is_bitwise_neutral(gpu, monkeypatch, dtype):
+    """The default tile choice (timed once per shape) gives bitwise the output
+    of the fixed 32x16x2 tile, including the in-place recurrence form
+    (out aliasing z) that the tuning must not apply twice."""
+    g, veff, nl, rng = small_case(n=(40, 35, 13), seed=50)
+    x = rng.uniform(-0.5, 0.5, (g.nd, 6))
+    z = rng.uniform(-0.5, 0.5, (g.nd, 6))
+    outs = {}
+    for tile in (None, "0"):
+        if tile is None:
+            monkeypatch.delenv("SPARC_AMD_STENCIL_TILE", raising=False)
+        else:
+            monkeypatch.setenv("SPARC_AMD_STENCIL_TILE", tile)
+        with gpu.Context(g) as ctx:
+            ctx.set_veff(veff)
+            bx = ctx.block(6, dtype).upload(x)
+            bz = ctx.block(6, dtype).upload(z)
+            ctx.chebyshev_filter_dev(bx, bz, 3, 2.0, 9.0, -1.0)
+            outs[tile] = (bx.download(), bz.download())
+            shape = ctx.stencil_tile(dtype, 6)
+        if tile is None:
+            assert shape in ((32, 16, 2), (32, 32, 2), (64, 16, 2))
+        else:
+            assert shape == (32, 16, 2)
+    assert np.array_equal(outs[None][0], outs["0"][0])
+    assert np.array_equal(outs[None][1], outs["0"][1])
+
+
 @pytest.mark.parametrize("cfg", [5, 6])
 @pytest.mark.parametrize("n,bc,ncol", [((40, 35, 13), (0, 0, 0), 3), ((70, 33, 9), (0, 1, 1), 2),
                                        ((7, 6, 30), (0, 0, 0), 1), ((33, 17, 40), (1, 0, 0), 1)])
