@@ -75,14 +75,16 @@ __global__ __launch_bounds__(256) void aar_gram(const double *__restrict__ F, lo
         partial[(long)blockIdx.x * NV + k] = sh[0][k] + sh[1][k] + sh[2][k] + sh[3][k];
 }
 
-__global__ void aar_gram_final(const double *__restrict__ partial, int nblk, int nv,
-                               double *__restrict__ out)
+// One wave per Gram value: lane l sums the partials l, l + 64, ... in order,
+// then a fixed butterfly over the wave (deterministic).
+__global__ __launch_bounds__(64) void aar_gram_final(const double *__restrict__ partial, int nblk,
+                                                     int nv, double *__restrict__ out)
 {
-    int k = threadIdx.x;
-    if (k >= nv) return;
+    const int k = blockIdx.x, lane = threadIdx.x;
     double s = 0.0;
-    for (int b = 0; b < nblk; b++) s += partial[(long)b * nv + k];
-    out[k] = s;
+    for (int b = lane; b < nblk; b += 64) s += partial[(long)b * nv + k];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) out[k] = s;
 }
 
 // x = x_old - X G + beta (f - F G)
@@ -184,7 +186,7 @@ int sgpu_anderson_coeff_dev(sgpu_context *ctx, const double *F, long ldf, int m,
     SGPU_CHECK(sgpu_reduce_buf(ctx, (size_t)gram_blocks * nv + nv, &partial));
     double *out = partial + (size_t)gram_blocks * nv;
     aar_gram<kAarMaxM><<<gram_blocks, 256, 0, ctx->stream>>>(F, ldf, f, m, n, partial);
-    aar_gram_final<<<1, 64, 0, ctx->stream>>>(partial, gram_blocks, nv, out);
+    aar_gram_final<<<nv, 64, 0, ctx->stream>>>(partial, gram_blocks, nv, out);
     SGPU_HIP(hipGetLastError());
     SGPU_HIP(hipMemcpyAsync(ctx->host_red, out, nv * sizeof(double), hipMemcpyDeviceToHost,
                             ctx->stream));

@@ -338,8 +338,10 @@ int gram_tn(sgpu_context *ctx, int dtype, int n, long nd, const void *A, long ld
     return gram_tn_typed<float>(ctx, n, nd, (const float *)A, lda, (const float *)B, ldb, C);
 }
 
-int project_raw(sgpu_context *ctx, int dtype, int ncol, const void *Y, void *HY, long ld,
-                double *Hp_host, double *Mp_host)
+// HY = H Y, then Mp = Y'Y at *dmat and Hp = Y'HY at *dmat + ncol^2 (FP64,
+// device, in the context's reduction buffer).
+int project_device(sgpu_context *ctx, int dtype, int ncol, const void *Y, void *HY, long ld,
+                   double **dmat_out)
 {
     sgpu_tmark t;
     SGPU_CHECK(apply_h(ctx, dtype, ncol, 0.0, Y, ld, HY, ld, nullptr, 0, 1.0, 0.0));
@@ -350,6 +352,16 @@ int project_raw(sgpu_context *ctx, int dtype, int ncol, const void *Y, void *HY,
     SGPU_CHECK(gram_tn(ctx, dtype, ncol, ctx->nd, Y, ld, Y, ld, dmat));
     SGPU_CHECK(gram_tn(ctx, dtype, ncol, ctx->nd, Y, ld, HY, ld, dmat + nn));
     SGPU_CHECK(sgpu_tend(ctx, &t, "project_gemm"));
+    *dmat_out = dmat;
+    return SGPU_OK;
+}
+
+int project_raw(sgpu_context *ctx, int dtype, int ncol, const void *Y, void *HY, long ld,
+                double *Hp_host, double *Mp_host)
+{
+    double *dmat = nullptr;
+    const long nn = (long)ncol * ncol;
+    SGPU_CHECK(project_device(ctx, dtype, ncol, Y, HY, ld, &dmat));
     SGPU_HIP(hipMemcpyAsync(Mp_host, dmat, nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     SGPU_HIP(hipMemcpyAsync(Hp_host, dmat + nn, nn * sizeof(double), hipMemcpyDeviceToHost,
                             ctx->stream));
@@ -357,15 +369,13 @@ int project_raw(sgpu_context *ctx, int dtype, int ncol, const void *Y, void *HY,
     return SGPU_OK;
 }
 
-int rotate_raw(sgpu_context *ctx, int dtype, int ns, const void *Y, long ldy, const double *Q_host,
-               void *out, long ldo)
+// out = Y Q with Q (ns x ns, FP64) on the device.
+int rotate_device(sgpu_context *ctx, int dtype, int ns, const void *Y, long ldy, const double *q64,
+                  void *out, long ldo)
 {
     sgpu_tmark t;
     const long nn = (long)ns * ns;
     void *qd = nullptr;
-    double *q64 = nullptr;
-    SGPU_CHECK(sgpu_reduce_buf(ctx, (size_t)nn, &q64));
-    SGPU_HIP(hipMemcpyAsync(q64, Q_host, nn * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     SGPU_CHECK(sgpu_tbegin(ctx, &t));
     SGPU_BLAS(rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host));
     if (dtype == SGPU_F64) {
@@ -383,29 +393,36 @@ int rotate_raw(sgpu_context *ctx, int dtype, int ns, const void *Y, long ldy, co
                                 (rocblas_int)ldo));
     }
     SGPU_CHECK(sgpu_tend(ctx, &t, "rotate_gemm"));
+    return SGPU_OK;
+}
+
+int rotate_raw(sgpu_context *ctx, int dtype, int ns, const void *Y, long ldy, const double *Q_host,
+               void *out, long ldo)
+{
+    const long nn = (long)ns * ns;
+    double *q64 = nullptr;
+    SGPU_CHECK(sgpu_reduce_buf(ctx, (size_t)nn, &q64));
+    SGPU_HIP(hipMemcpyAsync(q64, Q_host, nn * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    SGPU_CHECK(rotate_device(ctx, dtype, ns, Y, ldy, q64, out, ldo));
     SGPU_HIP(hipStreamSynchronize(ctx->stream));
     return SGPU_OK;
 }
 
-// dsygvd on the device (FP64).  Hp (host, n x n) is overwritten by Q.
+// dsygvd on the device (FP64), in place: dA (Hp, n x n) receives Q, dB (Mp)
+// its Cholesky factor; lambda (host) the eigenvalues in ascending order.
 // Hp Q = Mp Q Lambda with Q' Mp Q = I (LAPACKE_dsygvd itype 1 in the
 // reference, src/eigenSolver.c:1262-1375), solved as Mp = U'U (dpotrf),
 // C = U^-T Hp U^-1 (two dtrsm), C Z = Z Lambda (dsyevd), Q = U^-1 Z (dtrsm).
 // Same reduction as dsygvd, but rocSOLVER's dsygst took 16 ms of dsygvd's
 // 29 ms at n = 480 on MI355X where the two triangular solves take well under
-// one.  Hp receives Q (column-major), lambda the eigenvalues in ascending
-// order.
-int eigensolve(sgpu_context *ctx, int n, double *Hp, double *Mp, double *lambda)
+// one.
+int eigensolve_device(sgpu_context *ctx, int n, double *dA, double *dB, double *lambda)
 {
-    const long nn = (long)n * n;
     void *buf = nullptr;
-    // A, B, D, E, info in one scratch allocation
-    const size_t bytes = (2 * nn + 2 * (size_t)n) * sizeof(double) + 64;
-    SGPU_CHECK(sgpu_scratch(ctx, bytes, &buf));
-    double *dA = (double *)buf, *dB = dA + nn, *dD = dB + nn, *dE = dD + n;
+    // D, E, info in the shared scratch (free again once lambda is on the host)
+    SGPU_CHECK(sgpu_scratch(ctx, 2 * (size_t)n * sizeof(double) + 64, &buf));
+    double *dD = (double *)buf, *dE = dD + n;
     rocblas_int *dinfo = (rocblas_int *)(dE + n);
-    SGPU_HIP(hipMemcpyAsync(dA, Hp, nn * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    SGPU_HIP(hipMemcpyAsync(dB, Mp, nn * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     sgpu_tmark t;
     SGPU_CHECK(sgpu_tbegin(ctx, &t));
     rocblas_int info = 0;
@@ -431,7 +448,6 @@ int eigensolve(sgpu_context *ctx, int n, double *Hp, double *Mp, double *lambda)
                             rocblas_operation_none, rocblas_diagonal_non_unit, n, n, &one, dB, n,
                             dA, n));
     SGPU_CHECK(sgpu_tend(ctx, &t, "eigensolve"));
-    SGPU_HIP(hipMemcpyAsync(Hp, dA, nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     SGPU_HIP(hipMemcpyAsync(lambda, dD, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     SGPU_HIP(hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, ctx->stream));
     SGPU_HIP(hipStreamSynchronize(ctx->stream));
@@ -439,6 +455,20 @@ int eigensolve(sgpu_context *ctx, int n, double *Hp, double *Mp, double *lambda)
         sgpu_set_error("eigensolve: dsyevd did not converge (info = %d)", (int)info);
         return SGPU_ERR_SOLVER;
     }
+    return SGPU_OK;
+}
+
+// The same from host matrices: Hp (host) is overwritten by Q.
+int eigensolve(sgpu_context *ctx, int n, double *Hp, double *Mp, double *lambda)
+{
+    const long nn = (long)n * n;
+    double *dmat = nullptr;
+    SGPU_CHECK(sgpu_reduce_buf(ctx, (size_t)(2 * nn), &dmat));
+    SGPU_HIP(hipMemcpyAsync(dmat, Hp, nn * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    SGPU_HIP(hipMemcpyAsync(dmat + nn, Mp, nn * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    SGPU_CHECK(eigensolve_device(ctx, n, dmat, dmat + nn, lambda));
+    SGPU_HIP(hipMemcpyAsync(Hp, dmat, nn * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    SGPU_HIP(hipStreamSynchronize(ctx->stream));
     return SGPU_OK;
 }
 
@@ -574,11 +604,16 @@ extern "C" int sgpu_chefsi_dev(sgpu_context *ctx, sgpu_block *X, sgpu_block *Y, 
 {
     if (!ctx || !lambda) return SGPU_ERR_ARG;
     SGPU_CHECK(sgpu_chebyshev_filter_dev(ctx, X, Y, m, a, b, a0));
+    // projection, eigensolve and rotation on device-resident matrices: Mp
+    // and Hp stay in the reduction buffer, Hp becomes Q in place
     const int ns = Y->ncol;
-    std::vector<double> Hp((size_t)ns * ns), Mp((size_t)ns * ns);
-    SGPU_CHECK(sgpu_project_dev(ctx, Y, X, Hp.data(), Mp.data()));
-    SGPU_CHECK(eigensolve(ctx, ns, Hp.data(), Mp.data(), lambda));
-    return sgpu_rotate_dev(ctx, Y, Hp.data(), X);
+    const long nn = (long)ns * ns;
+    double *dmat = nullptr;
+    SGPU_CHECK(project_device(ctx, Y->dtype, ns, Y->ptr, X->ptr, Y->ld, &dmat));
+    SGPU_CHECK(eigensolve_device(ctx, ns, dmat + nn, dmat, lambda));
+    SGPU_CHECK(rotate_device(ctx, Y->dtype, ns, Y->ptr, Y->ld, dmat + nn, X->ptr, X->ld));
+    SGPU_HIP(hipStreamSynchronize(ctx->stream));
+    return SGPU_OK;
 }
 
 extern "C" int sgpu_lanczos(sgpu_context *ctx, const double *x0, double tol_min, double tol_max,

@@ -137,3 +137,4 @@ def test_unet_trained_preconditioner_beats_jacobi():
     xu -= xu.mean()
     assert np.linalg.norm(xu - xj) < 1e-6 * np.linalg.norm(xj)
     assert itu < itj
+
