@@ -60,6 +60,7 @@ __device__ __forceinline__ int wrap(int i, int n)
 __device__ void stage_weights(const float *__restrict__ W, int cin, int cout, float *lds)
 {
     const int total = cin * cout * 27;
+#pragma unroll 8
     for (int t = threadIdx.x; t < total; t += blockDim.x) {
         const int k = t % 27, ci = (t / 27) % cin, co = t / (27 * cin);
         lds[(k * cin + ci) * cout + co] = bfround(W[t]);
@@ -182,12 +183,35 @@ typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kTilesPerWave = 8;  // 16-voxel tiles per wave: amortises the weight staging
 
-// weights W[co][ci][27] -> LDS [k][co][ci] as BF16 bits
-__device__ void stage_weights_bf16(const float *__restrict__ W, uint16_t *lds)
+// C -> C weights packed once per pass as BF16 in the layout the MFMA B
+// operand reads, [k][co][ci] (pack_bf16), so that a block stages them with
+// 864 coalesced 16-byte copies.  (Converting W[co][ci][27] while staging
+// scattered 2-byte LDS stores 512 bytes apart: ~26 bank-conflict cycles per
+// LDS instruction, the largest cost of every layer.)
+constexpr int kWcc = 27 * kC * kC;  // weights of one C -> C layer
+
+__global__ void pack_bf16(const float *__restrict__ W, int nlayer, uint16_t *__restrict__ Wb)
 {
-    for (int t = threadIdx.x; t < 27 * kC * kC; t += blockDim.x) {
-        const int k = t % 27, ci = (t / 27) % kC, co = t / (27 * kC);
-        lds[(k * kC + co) * kC + ci] = f2bf(W[t]);
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nlayer * kWcc) return;
+    const int l = e / kWcc, u = e - l * kWcc;
+    const int k = u / (kC * kC), co = (u / kC) % kC, ci = u % kC;
+    Wb[e] = f2bf(W[(long)l * kWcc + (co * kC + ci) * 27 + k]);
+}
+
+// 256-thread blocks: 864 16-byte pieces, consecutive threads on consecutive
+// pieces (coalesced reads, conflict-free LDS writes)
+__device__ void stage_packed(const uint16_t *__restrict__ Wb, uint16_t *lds)
+{
+    constexpr int kPieces = kWcc / 8;  // 864 = 4 x 216
+    if (threadIdx.x < kPieces / 4) {
+        const uint4 *src = reinterpret_cast<const uint4 *>(Wb);
+        uint4 *dst = reinterpret_cast<uint4 *>(lds);
+        uint4 v[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) v[j] = src[threadIdx.x + j * (kPieces / 4)];
+#pragma unroll
+        for (int j = 0; j < 4; j++) dst[threadIdx.x + j * (kPieces / 4)] = v[j];
     }
     __syncthreads();
 }
@@ -202,11 +226,11 @@ __device__ __forceinline__ void store_tile(uint16_t *__restrict__ out, const uin
 template <int S>
 __global__ __launch_bounds__(256) void conv_cc_mfma(const uint16_t *__restrict__ in, Dims di,
                                                     uint16_t *__restrict__ out, Dims dout,
-                                                    const float *__restrict__ W,
+                                                    const uint16_t *__restrict__ Wb,
                                                     const uint16_t *__restrict__ skip, int tpw)
 {
     __shared__ __align__(16) uint16_t wl[27 * kC * kC];
-    stage_weights_bf16(W, wl);
+    stage_packed(Wb, wl);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, g = lane >> 4;
     const int txn = (dout.nx + 15) / 16;
@@ -244,15 +268,106 @@ __global__ __launch_bounds__(256) void conv_cc_mfma(const uint16_t *__restrict__
     }
 }
 
+// Stride-1 C -> C layer with the block's input staged in LDS: the block
+// computes 16 x 4 x 4 output voxels (16 tiles, 4 per wave) from an
+// 18 x 6 x 6-voxel input box loaded once (coalesced 16-byte pieces of the
+// channels-last rows), instead of each tile fetching its 27 taps through L2
+// (27 x the layer's input in L2 traffic).  Per tile the same 27 MFMAs with
+// the same operands as conv_cc_mfma, so the output is bitwise the same.
+// Staged coordinates are taken modulo the grid (periodic); lanes and rows
+// past the grid edge compute on wrapped data and are not stored.
+constexpr int kLX = 16, kLY = 4, kLZ = 4;
+constexpr int kLBX = kLX + 2, kLBY = kLY + 2, kLBZ = kLZ + 2;
+
+__device__ __forceinline__ int pmod(int i, int n)
+{
+    const int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+__global__ __launch_bounds__(256) void conv_cc_lds(const uint16_t *__restrict__ in, Dims d,
+                                                   uint16_t *__restrict__ out,
+                                                   const uint16_t *__restrict__ Wb,
+                                                   const uint16_t *__restrict__ skip)
+{
+    __shared__ __align__(16) uint16_t wl[27 * kC * kC];
+    // the input box, padded to a whole number of 16-byte pieces per thread
+    constexpr int kPieces = kLBZ * kLBY * kLBX * 2, kPer = (kPieces + 255) / 256;
+    __shared__ __align__(16) uint16_t xb[kPer * 256 * 8];
+    const int bxn = (d.nx + kLX - 1) / kLX, byn = (d.ny + kLY - 1) / kLY;
+    const int bx = blockIdx.x % bxn, by = (blockIdx.x / bxn) % byn, bz = blockIdx.x / (bxn * byn);
+    const int x0 = bx * kLX, y0 = by * kLY, z0 = bz * kLZ;
+    // input box: voxel v = (zz * kLBY + yy) * kLBX + xx, two 16-byte halves
+    // each; every load of a thread is issued before its first LDS store.
+    // Loads and stores are unconditional (clamped source, padded box): a
+    // guarded store let the compiler sink each load into its branch and wait
+    // for it there, one memory latency per piece.
+    // Swizzle: the 8-byte channel group q of box voxel xx sits in slot
+    // q ^ ((xx >> 2) & 3), so that 16 lanes reading group g of 16
+    // consecutive voxels hit 16 different bank pairs (unswizzled, the 32-byte
+    // voxel stride put them on 8 banks: a 4-way conflict on every A read).
+    uint4 pv[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+        const int c = min((int)threadIdx.x + 256 * j, kPieces - 1);
+        const int v = c >> 1, h = c & 1;
+        const int xx = v % kLBX, yy = (v / kLBX) % kLBY, zz = v / (kLBX * kLBY);
+        const long gv =
+            d.vox(pmod(x0 + xx - 1, d.nx), pmod(y0 + yy - 1, d.ny), pmod(z0 + zz - 1, d.nz));
+        pv[j] = *reinterpret_cast<const uint4 *>(in + gv * kC + 8 * h);
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+        const int c = (int)threadIdx.x + 256 * j;  // < kPer * 256: the box is padded
+        const int v = c >> 1, h = c & 1;
+        const int sw = ((v % kLBX) >> 2) & 3;
+        uint4 val = pv[j];
+        if (sw & 1) val = uint4{val.z, val.w, val.x, val.y};  // swap the two 8-byte groups
+        *reinterpret_cast<uint4 *>(xb + (2 * v + (h ^ (sw >> 1))) * 8) = val;
+    }
+    stage_packed(Wb, wl);  // ends with a barrier
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, g = lane >> 4;
+    // B operands (the 27 taps' weights) read once per wave; per tile the 27
+    // A reads are issued before the MFMA chain
+    bf16x4 bw[27];
+#pragma unroll
+    for (int k = 0; k < 27; k++) bw[k] = *reinterpret_cast<const bf16x4 *>(wl + (k * kC + li) * kC + 4 * g);
+#pragma unroll 1
+    for (int t = 0; t < 4; t++) {
+        const int row = wave * 4 + t;
+        const int ty = row % kLY, tz = row / kLY;
+        const int oy = y0 + ty, oz = z0 + tz;
+        bf16x4 av[27];
+#pragma unroll
+        for (int k = 0; k < 27; k++) {
+            const int dx = k % 3, dy = (k / 3) % 3, dz = k / 9;
+            const int xx = li + dx;
+            av[k] = *reinterpret_cast<const bf16x4 *>(
+                xb + (((tz + dz) * kLBY + ty + dy) * kLBX + xx) * kC + 4 * (g ^ ((xx >> 2) & 3)));
+        }
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < 27; k++) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[k], bw[k], acc, 0, 0, 0);
+        if (oy < d.ny && oz < d.nz) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int x = x0 + 4 * g + j;
+                if (x < d.nx) store_tile(out, skip, d.vox(x, oy, oz), li, acc[j]);
+            }
+        }
+    }
+}
+
 // Transposed stride-2 layer on matrix cores: a tile is 16 fine voxels of one
 // x-parity (fine x = 2 (c0 + i) + px), so all of them use the same taps.
 __global__ __launch_bounds__(256) void convT_cc_mfma(const uint16_t *__restrict__ in, Dims di,
                                                      uint16_t *__restrict__ out, Dims dout,
-                                                     const float *__restrict__ W,
+                                                     const uint16_t *__restrict__ Wb,
                                                      const uint16_t *__restrict__ skip, int tpw)
 {
     __shared__ __align__(16) uint16_t wl[27 * kC * kC];
-    stage_weights_bf16(W, wl);
+    stage_packed(Wb, wl);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, g = lane >> 4;
     const int txn = (di.nx + 15) / 16;
@@ -274,17 +389,18 @@ __global__ __launch_bounds__(256) void convT_cc_mfma(const uint16_t *__restrict_
     if (fz & 1) { kz[0] = 0; cz[0] = wrap((fz + 1) / 2, di.nz); kz[1] = 2; cz[1] = (fz - 1) / 2; }
     else { kz[0] = 1; cz[0] = fz / 2; }
     // up to 8 taps: loads first, then the MFMA chain
+    // unconditional loads (taps past nk* read the first tap again and are
+    // skipped below): a guarded load became a branch with a wait after it
     bf16x4 av[8];
     int kk[8];
 #pragma unroll
     for (int t = 0; t < 8; t++) {
         const int a = t >> 2, b = (t >> 1) & 1, e = t & 1;
-        kk[t] = -1;
-        if (a < nkz && b < nky && e < nkx) {
-            kk[t] = kz[a] * 9 + ky[b] * 3 + kx[e];
-            const int cx = wrap(c + ox[e], di.nx);
-            av[t] = *reinterpret_cast<const bf16x4 *>(in + di.vox(cx, cy[b], cz[a]) * kC + 4 * g);
-        }
+        const bool ok = a < nkz && b < nky && e < nkx;
+        const int aa = ok ? a : 0, bb = ok ? b : 0, ee = ok ? e : 0;
+        kk[t] = ok ? kz[aa] * 9 + ky[bb] * 3 + kx[ee] : -1;
+        const int cx = wrap(c + ox[ee], di.nx);
+        av[t] = *reinterpret_cast<const bf16x4 *>(in + di.vox(cx, cy[bb], cz[aa]) * kC + 4 * g);
     }
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -317,11 +433,18 @@ __global__ __launch_bounds__(256) void conv_1c(const TI *__restrict__ in, Dims d
     float acc[kC];
 #pragma unroll
     for (int c = 0; c < kC; c++) acc[c] = 0.f;
+    // all 27 tap loads issued before the accumulation (same order of sums)
+    TI xin[27];
+#pragma unroll
     for (int k = 0; k < 27; k++) {
         const int ix = wrap(2 * ox + k % 3 - 1, di.nx);
         const int iy = wrap(2 * oy + (k / 3) % 3 - 1, di.ny);
         const int iz = wrap(2 * oz + k / 9 - 1, di.nz);
-        const float xv = bfround((float)in[di.vox(ix, iy, iz)] * in_scale);
+        xin[k] = in[di.vox(ix, iy, iz)];
+    }
+#pragma unroll
+    for (int k = 0; k < 27; k++) {
+        const float xv = bfround((float)xin[k] * in_scale);
 #pragma unroll
         for (int co = 0; co < kC; co++) acc[co] += wl[k * kC + co] * xv;
     }
@@ -349,17 +472,36 @@ __global__ __launch_bounds__(256) void convT_c1_out(const uint16_t *__restrict__
     else { kys[0] = 1; cys[0] = fy / 2; }
     if (fz & 1) { kzs[0] = 0; czs[0] = wrap((fz + 1) / 2, di.nz); kzs[1] = 2; czs[1] = (fz - 1) / 2; }
     else { kzs[0] = 1; czs[0] = fz / 2; }
-    float u = 0.f;
-    for (int a = 0; a < nkz; a++)
-        for (int b = 0; b < nky; b++)
-            for (int c = 0; c < nkx; c++) {
-                const int k = kzs[a] * 9 + kys[b] * 3 + kxs[c];
-                float xv[kC];
-                load16(in + di.vox(cxs[c], cys[b], czs[a]) * kC, xv);
+    // up to 8 taps (a, b, c) in the loop order z, y, x: all loads first,
+    // then the sums in that order
+    const TI rv = r[v];
+    // unconditional loads (taps past nk* read the first tap again and are
+    // not summed): a guarded load became a branch with a wait after it
+    uint4 tv[8][2];
 #pragma unroll
-                for (int ci = 0; ci < kC; ci++) u += wl[k * kC + ci] * xv[ci];
+    for (int t = 0; t < 8; t++) {
+        const int a = (t >> 2) < nkz ? t >> 2 : 0, b = ((t >> 1) & 1) < nky ? (t >> 1) & 1 : 0,
+                  c = (t & 1) < nkx ? t & 1 : 0;
+        const uint16_t *p = in + di.vox(cxs[c], cys[b], czs[a]) * kC;
+        tv[t][0] = *reinterpret_cast<const uint4 *>(p);
+        tv[t][1] = *reinterpret_cast<const uint4 *>(p + 8);
+    }
+    float u = 0.f;
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        const int a = t >> 2, b = (t >> 1) & 1, c = t & 1;
+        if (a < nkz && b < nky && c < nkx) {
+            const int k = kzs[a] * 9 + kys[b] * 3 + kxs[c];
+            const uint32_t w[8] = {tv[t][0].x, tv[t][0].y, tv[t][0].z, tv[t][0].w,
+                                   tv[t][1].x, tv[t][1].y, tv[t][1].z, tv[t][1].w};
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                u += wl[k * kC + 2 * i] * __uint_as_float(w[i] << 16);
+                u += wl[k * kC + 2 * i + 1] * __uint_as_float(w[i] & 0xffff0000u);
             }
-    f[v] = (TO)(((double)r[v] + (double)u * out_scale) * inv_d0);
+        }
+    }
+    f[v] = (TO)(((double)rv + (double)u * out_scale) * inv_d0);
 }
 
 // Weight gradient G[a][b][k] = sum_y A[y][a] B[S y + k - 1][b] over the
@@ -524,6 +666,7 @@ __global__ __launch_bounds__(256) void diff_stats(const double *__restrict__ x,
 unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 
 bool g_unet_scalar = false;  // SPARC_AMD_UNET_SCALAR=1 selects the FP32-FMA layer kernels
+bool g_unet_lds = true;      // SPARC_AMD_UNET_LDS=0: stride-1 layers without the LDS input box
 
 // 16-voxel tiles per wave: enough to amortise the weight staging, few enough
 // to keep ~2048 blocks in flight (the quarter-resolution layers are small)
@@ -532,23 +675,29 @@ int tiles_per_wave(long tiles)
     return (int)std::max(1L, std::min<long>(kTilesPerWave, tiles / (4L * 2048)));
 }
 
+// W: FP32 [co][ci][27] (FMA kernels); Wb: the same packed by pack_bf16
+// (matrix-core kernels)
 template <int S>
 void launch_conv_cc(const uint16_t *in, Dims di, uint16_t *out, Dims dout, const float *W,
-                    const uint16_t *skip, hipStream_t s)
+                    const uint16_t *Wb, const uint16_t *skip, hipStream_t s)
 {
     if (g_unet_scalar) {
         conv_cc<S><<<nblk(dout.n()), 256, 0, s>>>(in, di, out, dout, W, skip);
+    } else if (S == 1 && g_unet_lds) {
+        const long nb = (long)((dout.nx + kLX - 1) / kLX) * ((dout.ny + kLY - 1) / kLY) *
+                        ((dout.nz + kLZ - 1) / kLZ);
+        conv_cc_lds<<<(unsigned)nb, 256, 0, s>>>(in, dout, out, Wb, skip);
     } else {
         const long tiles = (long)((dout.nx + 15) / 16) * dout.ny * dout.nz;
         const int tpw = tiles_per_wave(tiles);
         const long per = 4L * tpw;
-        conv_cc_mfma<S><<<(unsigned)((tiles + per - 1) / per), 256, 0, s>>>(in, di, out, dout, W, skip,
+        conv_cc_mfma<S><<<(unsigned)((tiles + per - 1) / per), 256, 0, s>>>(in, di, out, dout, Wb, skip,
                                                                          tpw);
     }
 }
 
 void launch_convT_cc(const uint16_t *in, Dims di, uint16_t *out, Dims dout, const float *W,
-                     const uint16_t *skip, hipStream_t s)
+                     const uint16_t *Wb, const uint16_t *skip, hipStream_t s)
 {
     if (g_unet_scalar) {
         convT_cc<<<nblk(dout.n()), 256, 0, s>>>(in, di, out, dout, W, skip);
@@ -556,7 +705,7 @@ void launch_convT_cc(const uint16_t *in, Dims di, uint16_t *out, Dims dout, cons
         const long tiles = (long)((di.nx + 15) / 16) * 2 * dout.ny * dout.nz;
         const int tpw = tiles_per_wave(tiles);
         const long per = 4L * tpw;
-        convT_cc_mfma<<<(unsigned)((tiles + per - 1) / per), 256, 0, s>>>(in, di, out, dout, W, skip,
+        convT_cc_mfma<<<(unsigned)((tiles + per - 1) / per), 256, 0, s>>>(in, di, out, dout, Wb, skip,
                                                                          tpw);
     }
 }
@@ -571,6 +720,8 @@ struct sgpu_unet {
     int nparam = 0;
     int off[kLayers + 1] = {0};
     float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr, *wt = nullptr;
+    uint16_t *wb = nullptr;   // layers 1..6 packed for the matrix cores (pack_bf16)
+    uint16_t *wtb = nullptr;  // wt packed likewise
     // activations (BF16 channels-last) and their gradients
     uint16_t *a1 = nullptr, *b1 = nullptr, *a2 = nullptr, *b2 = nullptr, *c2 = nullptr,
              *s1 = nullptr, *c1 = nullptr;
@@ -604,13 +755,16 @@ int forward(sgpu_unet *u, const TI *r, TO *f)
 {
     hipStream_t s = u->ctx->stream;
     const float *W = u->w;
+    // layers 1..6 (C -> C, contiguous in W) packed for the matrix cores
+    pack_bf16<<<nblk(6L * kWcc), 256, 0, s>>>(layer_w(u, W, 1), 6, u->wb);
+    auto wb = [&](int l) -> const uint16_t * { return u->wb + (long)(l - 1) * kWcc; };
     conv_1c<TI><<<nblk(u->d1.n()), 256, 0, s>>>(r, u->d0, u->a1, u->d1, layer_w(u, W, 0), 1.f);
-    launch_conv_cc<1>(u->a1, u->d1, u->b1, u->d1, layer_w(u, W, 1), nullptr, s);
-    launch_conv_cc<2>(u->b1, u->d1, u->a2, u->d2, layer_w(u, W, 2), nullptr, s);
-    launch_conv_cc<1>(u->a2, u->d2, u->b2, u->d2, layer_w(u, W, 3), nullptr, s);
-    launch_conv_cc<1>(u->b2, u->d2, u->c2, u->d2, layer_w(u, W, 4), nullptr, s);
-    launch_convT_cc(u->c2, u->d2, u->s1, u->d1, layer_w(u, W, 5), u->b1, s);
-    launch_conv_cc<1>(u->s1, u->d1, u->c1, u->d1, layer_w(u, W, 6), nullptr, s);
+    launch_conv_cc<1>(u->a1, u->d1, u->b1, u->d1, layer_w(u, W, 1), wb(1), nullptr, s);
+    launch_conv_cc<2>(u->b1, u->d1, u->a2, u->d2, layer_w(u, W, 2), wb(2), nullptr, s);
+    launch_conv_cc<1>(u->a2, u->d2, u->b2, u->d2, layer_w(u, W, 3), wb(3), nullptr, s);
+    launch_conv_cc<1>(u->b2, u->d2, u->c2, u->d2, layer_w(u, W, 4), wb(4), nullptr, s);
+    launch_convT_cc(u->c2, u->d2, u->s1, u->d1, layer_w(u, W, 5), wb(5), u->b1, s);
+    launch_conv_cc<1>(u->s1, u->d1, u->c1, u->d1, layer_w(u, W, 6), wb(6), nullptr, s);
     convT_c1_out<TI, TO><<<nblk(u->d0.n()), 256, 0, s>>>(u->c1, u->d1, u->d0, layer_w(u, W, 7), r,
                                                          1.f, 1.0 / u->d0abs, f);
     SGPU_HIP(hipGetLastError());
@@ -654,6 +808,8 @@ int backward(sgpu_unet *u, const float *r, const float *df)
     auto transform = [&](int l, int mode) {
         wtransform<<<nblk(layer_size(l)), 256, 0, s>>>(layer_w(u, W, l), kLayerCout[l], kLayerCin[l],
                                                         mode, wt);
+        if (kLayerCin[l] == kC && kLayerCout[l] == kC)
+            pack_bf16<<<nblk(kWcc), 256, 0, s>>>(wt, 1, u->wtb);
     };
     // f = (r + U) / |D0|  ->  dU = df / |D0|: folded into the P1 passes as a scale
     const float sU = (float)(1.0 / u->d0abs);
@@ -668,28 +824,28 @@ int backward(sgpu_unet *u, const float *r, const float *df)
     // C1 (conv C->C, L1): dW6 = sum_y dc1[y][co] s1[y+k-1][ci]
     SGPU_CHECK(wgrad_launch(u, 1, u->ga, u->d1, u->s1, kC, u->d1, 0, 6));
     transform(6, 0);
-    launch_conv_cc<1>(u->ga, u->d1, u->gb, u->d1, wt, nullptr, s);  // ds1
+    launch_conv_cc<1>(u->ga, u->d1, u->gb, u->d1, wt, u->wtb, nullptr, s);  // ds1
     // P2 (convT L2->L1): dW5[co][ci][k] = sum_y c2[y][ci] ds1[2y+k-1][co] -> transpose
     SGPU_CHECK(wgrad_launch(u, 2, u->c2, u->d2, u->gb, kC, u->d1, 1, 5));
     transform(5, 1);
-    launch_conv_cc<2>(u->gb, u->d1, u->gc, u->d2, wt, nullptr, s);  // dc2
+    launch_conv_cc<2>(u->gb, u->d1, u->gc, u->d2, wt, u->wtb, nullptr, s);  // dc2
     // M2 (conv L2): dW4 = sum dc2[y][co] b2[y+k-1][ci]
     SGPU_CHECK(wgrad_launch(u, 1, u->gc, u->d2, u->b2, kC, u->d2, 0, 4));
     transform(4, 0);
-    launch_conv_cc<1>(u->gc, u->d2, u->gd, u->d2, wt, nullptr, s);  // db2
+    launch_conv_cc<1>(u->gc, u->d2, u->gd, u->d2, wt, u->wtb, nullptr, s);  // db2
     // B2 (conv L2): dW3 = sum db2[y][co] a2[y+k-1][ci]
     SGPU_CHECK(wgrad_launch(u, 1, u->gd, u->d2, u->a2, kC, u->d2, 0, 3));
     transform(3, 0);
-    launch_conv_cc<1>(u->gd, u->d2, u->gc, u->d2, wt, nullptr, s);  // da2
+    launch_conv_cc<1>(u->gd, u->d2, u->gc, u->d2, wt, u->wtb, nullptr, s);  // da2
     // R2 (conv/2 L1->L2): dW2 = sum da2[y][co] b1[2y+k-1][ci]
     SGPU_CHECK(wgrad_launch(u, 2, u->gc, u->d2, u->b1, kC, u->d1, 0, 2));
     // db1 = ds1 (skip) + R2 adjoint(da2): transposed conv with W'[ci][co][k] = W2[co][ci][k]
     transform(2, 1);
-    launch_convT_cc(u->gc, u->d2, u->ga, u->d1, wt, u->gb, s);  // db1
+    launch_convT_cc(u->gc, u->d2, u->ga, u->d1, wt, u->wtb, u->gb, s);  // db1
     // B1 (conv L1): dW1 = sum db1[y][co] a1[y+k-1][ci]
     SGPU_CHECK(wgrad_launch(u, 1, u->ga, u->d1, u->a1, kC, u->d1, 0, 1));
     transform(1, 0);
-    launch_conv_cc<1>(u->ga, u->d1, u->gb, u->d1, wt, nullptr, s);  // da1
+    launch_conv_cc<1>(u->ga, u->d1, u->gb, u->d1, wt, u->wtb, nullptr, s);  // da1
     // R1 (conv/2 1->C, L0->L1): dW0[co][0][k] = sum_y da1[y][co] r[2y+k-1]
     SGPU_CHECK(wgrad_launch(u, 2, u->gb, u->d1, r, 1, u->d0, 0, 0));
     SGPU_HIP(hipGetLastError());
@@ -739,6 +895,8 @@ extern "C" int sgpu_unet_create(sgpu_context *ctx, int channels, unsigned long l
     {
         const char *env = getenv("SPARC_AMD_UNET_SCALAR");
         g_unet_scalar = env && env[0] == '1';
+        const char *lds = getenv("SPARC_AMD_UNET_LDS");
+        g_unet_lds = !(lds && lds[0] == '0');
     }
     sgpu_unet *u = new sgpu_unet();
     u->ctx = ctx;
@@ -752,7 +910,8 @@ extern "C" int sgpu_unet_create(sgpu_context *ctx, int channels, unsigned long l
     int st;
     if ((st = dalloc(u, &u->w, u->nparam)) || (st = dalloc(u, &u->g, u->nparam)) ||
         (st = dalloc(u, &u->m, u->nparam)) || (st = dalloc(u, &u->v, u->nparam)) ||
-        (st = dalloc(u, &u->wt, (size_t)kC * kC * 27)))
+        (st = dalloc(u, &u->wt, (size_t)kC * kC * 27)) || (st = dalloc(u, &u->wb, 6 * (size_t)kWcc)) ||
+        (st = dalloc(u, &u->wtb, (size_t)kWcc)))
         return fail(st);
     const size_t n1 = (size_t)u->d1.n() * kC, n2 = (size_t)u->d2.n() * kC;
     uint16_t **l1[] = {&u->a1, &u->b1, &u->s1, &u->c1, &u->ga, &u->gb};

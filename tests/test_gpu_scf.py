@@ -134,24 +134,30 @@ def test_scf_sic_mixed_matches_reference(sic):
 
 @pytest.fixture(scope="module")
 def si8u():
-    """8-atom Si on 28^3 (a grid the U-Net's two halvings accept), its FP64
-    Jacobi SCF and a U-Net trained on the device on (residual, error) pairs
-    of Jacobi-AAR solves of its Poisson problems."""
+    """8-atom Si on 28^3 (a grid the U-Net's two halvings accept) and its
+    FP64 Jacobi SCF, plus a U-Net trained on the device on (residual, error)
+    pairs of Jacobi-AAR Poisson solves of the 64-atom cell (52^3: on 28^3 the
+    quarter level is 7^3 and 400 steps do not train reliably), used across
+    grids as in the benchmark."""
     from sparc_amd import gpu
     from sparc_amd.systems import scf_system
     gpu.load()
-    st, p = scf_system("si8u")
-    nd = st.grid.nd
-    with gpu.Context(st.grid) as ctx:
-        ctx.set_nonlocal(st.nloc)
-        rj = ctx.scf_run(st.b, st.rho_at, dict(p, dtype=gpu.F64), rho_core=st.rho_core)
+    s64, p64 = scf_system("si64")
+    with gpu.Context(s64.grid) as ctx:
+        ctx.set_nonlocal(s64.nloc)
+        r2 = ctx.scf_run(s64.b, s64.rho_at, dict(p64, max_scf=2, dtype=gpu.F32),
+                         rho_core=s64.rho_core)
         net = ctx.unet(seed=1)
-        for rho in (st.rho_at, rj["rho"]):
-            net.harvest(4.0 * np.pi * (st.b + rho), np.zeros(nd), tol=1e-8, max_iter=3000,
-                        stride=2, max_pairs=100)
+        for rho in (s64.rho_at, r2["rho"]):
+            net.harvest(4.0 * np.pi * (s64.b + rho), np.zeros(s64.grid.nd), tol=1e-8,
+                        max_iter=3000, stride=2, max_pairs=120)
         hist = net.train(400, batch=4, lr=2e-3, seed=5)
         w = net.get_weights()
         net.close()
+    st, p = scf_system("si8u")
+    with gpu.Context(st.grid) as ctx:
+        ctx.set_nonlocal(st.nloc)
+        rj = ctx.scf_run(st.b, st.rho_at, dict(p, dtype=gpu.F64), rho_core=st.rho_core)
     return st, p, rj, w, hist
 
 
@@ -171,7 +177,7 @@ def test_scf_learned_poisson_preconditioner(si8u):
     iterations in total and no solve handed back to Jacobi."""
     from sparc_amd import gpu
     st, p, rj, w, hist = si8u
-    assert hist[-20:].mean() < 0.5 * hist[:5].mean()
+    assert hist[-20:].mean() < 0.2 * hist[:5].mean()
     ru = _learned_scf(st, dict(p, dtype=gpu.F64), w)
     assert rj["converged"] and ru["converged"]
     assert ru["poisson_fallbacks"] == 0

@@ -138,3 +138,25 @@ def test_unet_trained_preconditioner_beats_jacobi():
     assert np.linalg.norm(xu - xj) < 1e-6 * np.linalg.norm(xj)
     assert itu < itj
 
+
+
+def test_unet_lds_staged_layers_match_direct_taps(monkeypatch):
+    """The stride-1 layers that stage their input box in LDS give bitwise the
+    output of the kernel that fetches every tap from memory (the same MFMAs on
+    the same operands), on a grid whose levels leave partial x, y and z
+    tiles (24 x 20 x 12 -> 12 x 10 x 6 -> 6 x 5 x 3)."""
+    from sparc_amd import gpu
+    from sparc_amd.workloads import Grid
+    g = Grid(24, 20, 12, 0.4, 0.4, 0.4)
+    r = np.random.default_rng(21).standard_normal(g.nd)
+    out = {}
+    for lds in ("1", "0"):
+        monkeypatch.setenv("SPARC_AMD_UNET_LDS", lds)
+        with gpu.Context(g) as ctx:
+            net = ctx.unet(seed=5)
+            w = net.get_weights()
+            net.set_weights(w + np.random.default_rng(6).standard_normal(w.size).astype(np.float32)
+                            * 0.05)
+            out[lds] = net.apply(r)
+    assert np.isfinite(out["1"]).all()
+    assert np.array_equal(out["1"], out["0"])
