@@ -97,23 +97,48 @@ __global__ __launch_bounds__(256) void nlb_project(const T *__restrict__ X, long
     const int col0 = (lid % gridDim.x) * NC;
     const int x0 = brick_org[3 * b], y0 = brick_org[3 * b + 1], z0 = brick_org[3 * b + 2];
     constexpr int VW = sizeof(typename M::vec) / sizeof(T);
+    // All loads of a thread are issued before its first LDS store, from
+    // clamped addresses (X[0] for points outside the grid or columns past
+    // ncol, zeroed after the load): a guarded load became a branch that
+    // waited for each load in turn, one memory latency per element.
     if (nx % VW == 0 && ldx % VW == 0) {
         // 16-byte loads: a vector never straddles the grid edge because
         // x0 is a multiple of kBX and nx a multiple of VW
-#pragma unroll 4
-        for (int e = threadIdx.x; e < kBP * NC / VW; e += 256) {
+        constexpr int kPer = kBP * NC / VW / 256;
+        typename M::vec v[kPer];
+        bool ok[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; j++) {
+            const int e = threadIdx.x + 256 * j;
             const int c = e / (kBP / VW), q = (e - c * (kBP / VW)) * VW;
             const long pt = brick_point(q, x0, y0, z0, nx, ny, nz);
-            typename M::vec v{};
-            if (pt >= 0 && col0 + c < ncol)
-                v = *reinterpret_cast<const typename M::vec *>(X + (long)(col0 + c) * ldx + pt);
-            *reinterpret_cast<typename M::vec *>(xs + c * kLDQ + q) = v;
+            ok[j] = pt >= 0 && col0 + c < ncol;
+            v[j] = *reinterpret_cast<const typename M::vec *>(X + (ok[j] ? (long)(col0 + c) * ldx + pt : 0L));
+        }
+#pragma unroll
+        for (int j = 0; j < kPer; j++) {
+            const int e = threadIdx.x + 256 * j;
+            const int c = e / (kBP / VW), q = (e - c * (kBP / VW)) * VW;
+            if (!ok[j]) v[j] = typename M::vec{};
+            *reinterpret_cast<typename M::vec *>(xs + c * kLDQ + q) = v[j];
         }
     } else {
-        for (int e = threadIdx.x; e < kBP * NC; e += 256) {
+        constexpr int kPer = kBP * NC / 256;
+        T v[kPer];
+        bool ok[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; j++) {
+            const int e = threadIdx.x + 256 * j;
             const int c = e / kBP, q = e - c * kBP;
             const long pt = brick_point(q, x0, y0, z0, nx, ny, nz);
-            xs[c * kLDQ + q] = (pt >= 0 && col0 + c < ncol) ? X[(long)(col0 + c) * ldx + pt] : T(0);
+            ok[j] = pt >= 0 && col0 + c < ncol;
+            v[j] = X[ok[j] ? (long)(col0 + c) * ldx + pt : 0L];
+        }
+#pragma unroll
+        for (int j = 0; j < kPer; j++) {
+            const int e = threadIdx.x + 256 * j;
+            const int c = e / kBP, q = e - c * kBP;
+            xs[c * kLDQ + q] = ok[j] ? v[j] : T(0);
         }
     }
     __syncthreads();
@@ -227,10 +252,26 @@ __global__ __launch_bounds__(256) void nlb_apply(T *__restrict__ out, long ldo, 
     const int x0 = brick_org[3 * b], y0 = brick_org[3 * b + 1], z0 = brick_org[3 * b + 2];
     const int np = brick_np[b];
     const long row0 = brick_row[b];
-    for (int e = threadIdx.x; e < np * NC; e += 256) {
-        const int j = e / NC, c = e - j * NC;
-        const int r = row_alpha[row0 + j];
-        beta[j * LDB + c] = (r >= 0 && col0 + c < ncol) ? alpha_s[(long)r * ncol + col0 + c] : T(0);
+    // beta gathered four elements per thread at a time, every load issued
+    // before the stores (clamped indices: the tail repeats the last element)
+    for (int base = 0; base < np * NC; base += 4 * 256) {
+        int jj[4], cc[4], rr[4];
+        T vv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int e = min(base + (int)threadIdx.x + 256 * u, np * NC - 1);
+            jj[u] = e / NC;
+            cc[u] = e - jj[u] * NC;
+            rr[u] = row_alpha[row0 + jj[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const bool ok = rr[u] >= 0 && col0 + cc[u] < ncol;
+            vv[u] = alpha_s[ok ? (long)rr[u] * ncol + col0 + cc[u] : 0L];
+            if (!ok) vv[u] = T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) beta[jj[u] * LDB + cc[u]] = vv[u];
     }
     __syncthreads();
     const T *pc = phic + row0 * kBP;
@@ -329,11 +370,26 @@ __global__ __launch_bounds__(256) void nls_project(const T *__restrict__ X, long
         for (int ct = 0; ct < NCT; ct++) acc[rt][ct] = typename M::acc{0, 0, 0, 0};
     for (int qc = 0; qc < qpad; qc += kQC) {
         const int qn = min(kQC, qpad - qc);
-#pragma unroll 8
-        for (int t = threadIdx.x; t < qn * NC; t += 256) {
-            const int c = t / qn, q = t - c * qn;
-            const int g = gp[qc + q];
-            xs[c * kQLD + q] = (g >= 0 && col0 + c < ncol) ? X[(long)(col0 + c) * ldx + g] : T(0);
+        // eight elements per thread at a time: point indices, then values,
+        // then stores (clamped: the tail repeats the last element)
+        for (int base = 0; base < qn * NC; base += 8 * 256) {
+            int cc[8], qq[8], gg[8];
+            T vv[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int t = min(base + (int)threadIdx.x + 256 * u, qn * NC - 1);
+                cc[u] = t / qn;
+                qq[u] = t - cc[u] * qn;
+                gg[u] = gp[qc + qq[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const bool ok = gg[u] >= 0 && col0 + cc[u] < ncol;
+                vv[u] = X[ok ? (long)(col0 + cc[u]) * ldx + gg[u] : 0L];
+                if (!ok) vv[u] = T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) xs[cc[u] * kQLD + qq[u]] = vv[u];
         }
         __syncthreads();
         const int qw = qn / 4;  // a multiple of 16
@@ -414,14 +470,36 @@ __global__ __launch_bounds__(256) void nls_apply(T *__restrict__ out, long ldo, 
     const int qpad = ent_info[6 * e], poff = ent_info[6 * e + 1];
     const int np = ent_info[6 * e + 2], npad = ent_info[6 * e + 3], atom = ent_info[6 * e + 4];
     const int i0 = atom_img_ptr[atom], i1 = atom_img_ptr[atom + 1], row = atom_row[atom];
-    for (int t = threadIdx.x; t < npad * NC; t += 256) {
-        const int p = t / NC, c = t - p * NC;
-        T v = T(0);
-        if (p < np && col0 + c < ncol) {
-            for (int i = i0; i < i1; i++) v += part[((long)atom_img[i] * max_np + p) * ncol + col0 + c];
-            v *= (T)(scale_dv * gamma[row + p]);
+    // beta = scale dV Gamma sum over the atom's images of part, four
+    // elements per thread at a time with every load before the stores
+    // (clamped: the tail repeats the last element)
+    {
+        const int img0 = atom_img[i0];
+        for (int base = 0; base < npad * NC; base += 4 * 256) {
+            int pp[4], cc[4];
+            bool ok[4];
+            T vv[4];
+            double gm[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int t = min(base + (int)threadIdx.x + 256 * u, npad * NC - 1);
+                pp[u] = t / NC;
+                cc[u] = t - pp[u] * NC;
+                ok[u] = pp[u] < np && col0 + cc[u] < ncol;
+                vv[u] = part[ok[u] ? ((long)img0 * max_np + pp[u]) * ncol + col0 + cc[u] : 0L];
+                gm[u] = gamma[row + (ok[u] ? pp[u] : 0)];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                T v = ok[u] ? vv[u] : T(0);
+                if (ok[u]) {
+                    for (int i = i0 + 1; i < i1; i++)
+                        v += part[((long)atom_img[i] * max_np + pp[u]) * ncol + col0 + cc[u]];
+                    v *= (T)(scale_dv * gm[u]);
+                }
+                beta[pp[u] * LDB + cc[u]] = v;
+            }
         }
-        beta[p * LDB + c] = v;
     }
     __syncthreads();
     const T *ct_ = chit + ent_chi[e];
@@ -504,14 +582,36 @@ __global__ __launch_bounds__(256) void nls_apply_proj(T *__restrict__ out, long 
     const int np = ent_info[6 * e + 2], npad = ent_info[6 * e + 3], atom = ent_info[6 * e + 4];
     const int nrt = npad / 16;
     const int i0 = atom_img_ptr[atom], i1 = atom_img_ptr[atom + 1], row = atom_row[atom];
-    for (int t = threadIdx.x; t < npad * NC; t += 256) {
-        const int p = t / NC, c = t - p * NC;
-        T v = T(0);
-        if (p < np && col0 + c < ncol) {
-            for (int i = i0; i < i1; i++) v += part[((long)atom_img[i] * max_np + p) * ncol + col0 + c];
-            v *= (T)(scale_dv * gamma[row + p]);
+    // beta = scale dV Gamma sum over the atom's images of part, four
+    // elements per thread at a time with every load before the stores
+    // (clamped: the tail repeats the last element)
+    {
+        const int img0 = atom_img[i0];
+        for (int base = 0; base < npad * NC; base += 4 * 256) {
+            int pp[4], cc[4];
+            bool ok[4];
+            T vv[4];
+            double gm[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int t = min(base + (int)threadIdx.x + 256 * u, npad * NC - 1);
+                pp[u] = t / NC;
+                cc[u] = t - pp[u] * NC;
+                ok[u] = pp[u] < np && col0 + cc[u] < ncol;
+                vv[u] = part[ok[u] ? ((long)img0 * max_np + pp[u]) * ncol + col0 + cc[u] : 0L];
+                gm[u] = gamma[row + (ok[u] ? pp[u] : 0)];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                T v = ok[u] ? vv[u] : T(0);
+                if (ok[u]) {
+                    for (int i = i0 + 1; i < i1; i++)
+                        v += part[((long)atom_img[i] * max_np + pp[u]) * ncol + col0 + cc[u]];
+                    v *= (T)(scale_dv * gm[u]);
+                }
+                beta[pp[u] * LDB + cc[u]] = v;
+            }
         }
-        beta[p * LDB + c] = v;
     }
     __syncthreads();
     const T *ct_ = chit + ent_chi[e];
