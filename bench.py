@@ -49,14 +49,15 @@ def stencil_bytes(nd, ns, m, s):
     return s * nd * ns * (2 + 3 * (m - 1)) + m * s * nd
 
 
-def profiled_traffic(kernel="stencil_tiled"):
-    """HBM bytes per launch of the dominant kernel, from the committed
-    FETCH_SIZE / WRITE_SIZE summary (profiles/), or None."""
+def profiled_traffic(kernel):
+    """HBM bytes per launch of the dominant kernel (by its full name, e.g.
+    stencil_tiled<float,32,16,2>), from the committed FETCH_SIZE / WRITE_SIZE
+    summary (profiles/), or None."""
     import re
     p = ROOT / "profiles" / "filter_operator_96x512_deg24_hbm_traffic.md"
     if not p.exists():
         return None
-    m = re.search(rf"traffic_bytes_per_launch {kernel}:\s*([0-9.eE+]+)", p.read_text())
+    m = re.search(rf"traffic_bytes_per_launch {re.escape(kernel)}:\s*([0-9.eE+]+)", p.read_text())
     return float(m.group(1)) if m else None
 
 
@@ -356,6 +357,7 @@ def main(argv=None):
     print(f"[bench] fp32 filter {ms32:.2f} ms ({gbs32:.0f} GB/s), stencil launch {st_ms:.3f} ms "
           f"({st_gbs:.0f} GB/s), nonlocal {r32['nonlocal'][0] / max(r32['nonlocal'][1], 1):.3f} ms",
           flush=True)
+    kernel = "stencil_tiled<float,%d,%d,%d>" % ctx.stencil_tile(gpu.F32, ns)
     res = {
         "metric": METRIC,
         "value": round(gbs32, 2),
@@ -378,8 +380,9 @@ def main(argv=None):
         "filter_gbs": {"f32": round(gbs32, 2)},
         "roofline": {"bound": "hbm", "achieved": round(st_gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(st_gbs / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic() if (args.n, ns, m) == (96, 512, 24) else None,
-                     "kernel": "stencil_tiled<float,%d,%d,%d>" % ctx.stencil_tile(gpu.F32, ns),
+                     "traffic": (profiled_traffic(kernel)
+                                 if (args.n, ns, m) == (96, 512, 24) else None),
+                     "kernel": kernel,
                      "per_launch_algorithmic_bytes": stencil_bytes(nd, ns, m, 4) / m},
         "finite": r32["finite"],
     }

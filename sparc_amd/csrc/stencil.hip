@@ -475,8 +475,10 @@ int tune_tile(sgpu_context *ctx, StencilParams<T> p, int ncol, int *best_cfg)
                 st = SGPU_ERR_HIP;
                 break;
             }
-            if (rep == 1 && (c == cands[0] || ms < best)) {
-                best = ms;
+            // a shape must beat the current best by 3 %, so that timing noise
+            // between near-equal shapes does not change the choice run to run
+            if (rep == 1 && (c == cands[0] || ms < 0.97f * best)) {
+                best = c == cands[0] ? ms : std::min(best, ms);
                 *best_cfg = c;
             }
         }
