@@ -411,13 +411,14 @@ void launch_ring(sgpu_context *ctx, const StencilParams<T> &p, int ncol)
     stencil_ring<T, TX, TY, RY, NS><<<grid, TX * (TY / RY), 0, ctx->stream>>>(p, zchunk);
 }
 
+// min_chunk: fewest z-planes per block (each chunk re-reads 2 x 6 planes)
 template <typename T, int TX, int TY, int RY>
-void launch_tiled(sgpu_context *ctx, const StencilParams<T> &p, int ncol)
+void launch_tiled(sgpu_context *ctx, const StencilParams<T> &p, int ncol, int min_chunk = 16)
 {
     const int tiles = ((ctx->nx + TX - 1) / TX) * ((ctx->ny + TY - 1) / TY);
     // enough blocks to fill the chip: split z when there are few columns
     long want = (2048 + (long)tiles * ncol - 1) / ((long)tiles * ncol);
-    int nzc = (int)std::max(1L, std::min<long>(want, std::max(1, ctx->nz / 16)));
+    int nzc = (int)std::max(1L, std::min<long>(want, std::max(1, ctx->nz / min_chunk)));
     int zchunk = (ctx->nz + nzc - 1) / nzc;
     nzc = (ctx->nz + zchunk - 1) / zchunk;
     dim3 grid(tiles, ncol, nzc);
@@ -442,6 +443,8 @@ void launch_cfg(sgpu_context *ctx, const StencilParams<T> &p, int ncol, int cfg)
         case 4: launch_tiled<T, 64, 32, 8>(ctx, p, ncol); break;
         case 7: launch_tiled<T, 32, 32, 2>(ctx, p, ncol); break;
         case 8: launch_tiled<T, 64, 16, 2>(ctx, p, ncol); break;
+        case 9: launch_tiled<T, 32, 16, 2>(ctx, p, ncol, 8); break;
+        case 10: launch_tiled<T, 32, 16, 2>(ctx, p, ncol, 4); break;
         default: launch_tiled<T, 32, 16, 2>(ctx, p, ncol); break;
     }
 }
@@ -455,7 +458,11 @@ void launch_cfg(sgpu_context *ctx, const StencilParams<T> &p, int ncol, int cfg)
 template <typename T>
 int tune_tile(sgpu_context *ctx, StencilParams<T> p, int ncol, int *best_cfg)
 {
-    static const int cands[] = {0, 7, 8};
+    // few columns (the Poisson / Kerker residual, Lanczos): also thinner z-chunks
+    static const int cands_many[] = {0, 7, 8}, cands_few[] = {0, 7, 8, 9, 10};
+    const bool few = ncol < 16;
+    const int *cands = few ? cands_few : cands_many;
+    const int ncand = few ? 5 : 3;
     SGPU_HIP(hipStreamSynchronize(ctx->stream));
     T *tmp = nullptr;
     SGPU_HIP(hipMalloc((void **)&tmp, (size_t)ncol * p.ldo * sizeof(T)));
@@ -466,7 +473,8 @@ int tune_tile(sgpu_context *ctx, StencilParams<T> p, int ncol, int *best_cfg)
     *best_cfg = 0;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) st = SGPU_ERR_HIP;
     for (int rep = 0; rep < 2 && st == SGPU_OK; rep++) {
-        for (int c : cands) {
+        for (int ci = 0; ci < ncand; ci++) {
+            const int c = cands[ci];
             float ms = 0.f;
             if (hipEventRecord(e0, ctx->stream) != hipSuccess) { st = SGPU_ERR_HIP; break; }
             launch_cfg<T>(ctx, p, ncol, c);
@@ -578,9 +586,10 @@ extern "C" int sgpu_stencil_tile(const sgpu_context *ctx, int dtype, int ncol, i
         }
         cfg = it->second;
     }
-    static const int shape[9][3] = {{32, 16, 2}, {32, 32, 4}, {64, 16, 4}, {32, 32, 8}, {64, 32, 8},
-                                    {32, 16, 2}, {32, 8, 2},  {32, 32, 2}, {64, 16, 2}};
-    if (cfg > 8 || ((cfg == 5 || cfg == 6) && dtype == SGPU_F64)) cfg = 0;
+    static const int shape[11][3] = {{32, 16, 2}, {32, 32, 4}, {64, 16, 4}, {32, 32, 8},
+                                     {64, 32, 8}, {32, 16, 2}, {32, 8, 2},  {32, 32, 2},
+                                     {64, 16, 2}, {32, 16, 2}, {32, 16, 2}};
+    if (cfg > 10 || ((cfg == 5 || cfg == 6) && dtype == SGPU_F64)) cfg = 0;
     *tx = shape[cfg][0];
     *ty = shape[cfg][1];
     *ry = shape[cfg][2];
