@@ -1,7 +1,9 @@
 """Writes sic8_refout.json: the energies the reference's own test
 SiC_orthogonal_quick_relax (standard accuracy) prints for its first SCF
 (tests/SiC_orthogonal_quick_relax/standard/SiC_orthogonal_quick_relax.refout),
-together with the input it ran (cell, grid, atoms, settings).
+together with the input it ran (cell, grid, atoms, settings) and the
+effective input parameters it echoes; and copies the test's .inpt / .ion
+input files (data) to sic8/.
 
 Run in a container that holds the reference tree:
     python tests/golden/make_sic8_fixture.py /root/reference
@@ -49,6 +51,15 @@ fix = {
     "fermi_level": grab("Fermi level"),
     "scf_history": hist,
 }
-dst = Path(__file__).resolve().parent / "sic8_refout.json"
+# effective parameters echoed under "Input parameters" (the defaults it applied)
+head = first.split("Input parameters")[1].split("Cell")[0]
+fix["echo"] = {k: v.strip() for k, v in re.findall(r"^([A-Z_0-9]+):[ \t]*(.*)$", head, re.M)
+               if k not in ("LATVEC",)}
+here = Path(__file__).resolve().parent
+(here / "sic8").mkdir(exist_ok=True)
+for ext in (".inpt", ".ion"):
+    src = case / ("SiC_orthogonal_quick_relax" + ext)
+    (here / "sic8" / src.name).write_text(src.read_text())
+dst = here / "sic8_refout.json"
 dst.write_text(json.dumps(fix, indent=1) + "\n")
 print(f"wrote {dst}")
