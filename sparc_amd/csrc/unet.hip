@@ -453,55 +453,71 @@ __global__ __launch_bounds__(256) void conv_1c(const TI *__restrict__ in, Dims d
 
 // C -> 1 transposed stride-2 convolution to the fine level fused with the
 // preconditioner output: U = P1(in); f = (r + U * out_scale) / |D0|.
+// One thread per coarse voxel c writes the 2 x 2 x 2 fine voxels
+// 2c + {0,1}^3: each takes its taps from the coarse voxels c + {0,1}^3 only
+// (per axis, fine 2c: tap 1 of c; fine 2c+1: tap 0 of c+1, then tap 2 of c),
+// so the eight 32-byte coarse voxels are loaded once for eight outputs
+// instead of once per tap of every output.  Taps summed z, y, x outer to
+// inner, channels in order.
 template <typename TI, typename TO>
-__global__ __launch_bounds__(256) void convT_c1_out(const uint16_t *__restrict__ in, Dims di,
-                                                    Dims dout, const float *__restrict__ W,
-                                                    const TI *__restrict__ r, float out_scale,
-                                                    double inv_d0, TO *__restrict__ f)
+__global__ __launch_bounds__(256) void convT_c1_out_blk(const uint16_t *__restrict__ in, Dims di,
+                                                        Dims dout, const float *__restrict__ W,
+                                                        const TI *__restrict__ r, float out_scale,
+                                                        double inv_d0, TO *__restrict__ f)
 {
     __shared__ float wl[27 * kC];
     stage_weights(W, kC, 1, wl);
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= dout.n()) return;
-    const int fx = (int)(v % dout.nx), fy = (int)((v / dout.nx) % dout.ny), fz = (int)(v / ((long)dout.nx * dout.ny));
-    int kxs[2], cxs[2], kys[2], cys[2], kzs[2], czs[2];
-    const int nkx = (fx & 1) ? 2 : 1, nky = (fy & 1) ? 2 : 1, nkz = (fz & 1) ? 2 : 1;
-    if (fx & 1) { kxs[0] = 0; cxs[0] = wrap((fx + 1) / 2, di.nx); kxs[1] = 2; cxs[1] = (fx - 1) / 2; }
-    else { kxs[0] = 1; cxs[0] = fx / 2; }
-    if (fy & 1) { kys[0] = 0; cys[0] = wrap((fy + 1) / 2, di.ny); kys[1] = 2; cys[1] = (fy - 1) / 2; }
-    else { kys[0] = 1; cys[0] = fy / 2; }
-    if (fz & 1) { kzs[0] = 0; czs[0] = wrap((fz + 1) / 2, di.nz); kzs[1] = 2; czs[1] = (fz - 1) / 2; }
-    else { kzs[0] = 1; czs[0] = fz / 2; }
-    // up to 8 taps (a, b, c) in the loop order z, y, x: all loads first,
-    // then the sums in that order
-    const TI rv = r[v];
-    // unconditional loads (taps past nk* read the first tap again and are
-    // not summed): a guarded load became a branch with a wait after it
-    uint4 tv[8][2];
+    if (v >= di.n()) return;
+    const int cx = (int)(v % di.nx), cy = (int)((v / di.nx) % di.ny), cz = (int)(v / ((long)di.nx * di.ny));
+    // coarse voxel (cx + dx, cy + dy, cz + dz), d in {0,1}: 16 BF16 channels as 8 words
+    uint32_t cw[8][8];
 #pragma unroll
     for (int t = 0; t < 8; t++) {
-        const int a = (t >> 2) < nkz ? t >> 2 : 0, b = ((t >> 1) & 1) < nky ? (t >> 1) & 1 : 0,
-                  c = (t & 1) < nkx ? t & 1 : 0;
-        const uint16_t *p = in + di.vox(cxs[c], cys[b], czs[a]) * kC;
-        tv[t][0] = *reinterpret_cast<const uint4 *>(p);
-        tv[t][1] = *reinterpret_cast<const uint4 *>(p + 8);
+        const int dx = t & 1, dy = (t >> 1) & 1, dz = t >> 2;
+        const uint16_t *p = in + di.vox(wrap(cx + dx, di.nx), wrap(cy + dy, di.ny), wrap(cz + dz, di.nz)) * kC;
+        const uint4 a = *reinterpret_cast<const uint4 *>(p);
+        const uint4 b = *reinterpret_cast<const uint4 *>(p + 8);
+        cw[t][0] = a.x; cw[t][1] = a.y; cw[t][2] = a.z; cw[t][3] = a.w;
+        cw[t][4] = b.x; cw[t][5] = b.y; cw[t][6] = b.z; cw[t][7] = b.w;
     }
-    float u = 0.f;
+    TI rv[8];
 #pragma unroll
-    for (int t = 0; t < 8; t++) {
-        const int a = t >> 2, b = (t >> 1) & 1, c = t & 1;
-        if (a < nkz && b < nky && c < nkx) {
-            const int k = kzs[a] * 9 + kys[b] * 3 + kxs[c];
-            const uint32_t w[8] = {tv[t][0].x, tv[t][0].y, tv[t][0].z, tv[t][0].w,
-                                   tv[t][1].x, tv[t][1].y, tv[t][1].z, tv[t][1].w};
+    for (int o = 0; o < 8; o++) {
+        const int px = o & 1, py = (o >> 1) & 1, pz = o >> 2;
+        rv[o] = r[dout.vox(2 * cx + px, 2 * cy + py, 2 * cz + pz)];
+    }
 #pragma unroll
-            for (int i = 0; i < 8; i++) {
-                u += wl[k * kC + 2 * i] * __uint_as_float(w[i] << 16);
-                u += wl[k * kC + 2 * i + 1] * __uint_as_float(w[i] & 0xffff0000u);
+    for (int o = 0; o < 8; o++) {
+        const int px = o & 1, py = (o >> 1) & 1, pz = o >> 2;
+        // per axis: (tap, coarse offset) pairs, odd parity tap 0 then tap 2
+        const int nkx = px ? 2 : 1, nky = py ? 2 : 1, nkz = pz ? 2 : 1;
+        float u = 0.f;
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+            if (a >= nkz) continue;
+            const int kz = pz ? (a == 0 ? 0 : 2) : 1, dz = pz ? (a == 0 ? 1 : 0) : 0;
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                if (b >= nky) continue;
+                const int ky = py ? (b == 0 ? 0 : 2) : 1, dy = py ? (b == 0 ? 1 : 0) : 0;
+#pragma unroll
+                for (int c = 0; c < 2; c++) {
+                    if (c >= nkx) continue;
+                    const int kx = px ? (c == 0 ? 0 : 2) : 1, dx = px ? (c == 0 ? 1 : 0) : 0;
+                    const int k = kz * 9 + ky * 3 + kx;
+                    const uint32_t *w = cw[dz * 4 + dy * 2 + dx];
+#pragma unroll
+                    for (int i = 0; i < 8; i++) {
+                        u += wl[k * kC + 2 * i] * __uint_as_float(w[i] << 16);
+                        u += wl[k * kC + 2 * i + 1] * __uint_as_float(w[i] & 0xffff0000u);
+                    }
+                }
             }
         }
+        f[dout.vox(2 * cx + px, 2 * cy + py, 2 * cz + pz)] =
+            (TO)(((double)rv[o] + (double)u * out_scale) * inv_d0);
     }
-    f[v] = (TO)(((double)rv + (double)u * out_scale) * inv_d0);
 }
 
 // Weight gradient G[a][b][k] = sum_y A[y][a] B[S y + k - 1][b] over the
@@ -765,7 +781,7 @@ int forward(sgpu_unet *u, const TI *r, TO *f)
     launch_conv_cc<1>(u->b2, u->d2, u->c2, u->d2, layer_w(u, W, 4), wb(4), nullptr, s);
     launch_convT_cc(u->c2, u->d2, u->s1, u->d1, layer_w(u, W, 5), wb(5), u->b1, s);
     launch_conv_cc<1>(u->s1, u->d1, u->c1, u->d1, layer_w(u, W, 6), wb(6), nullptr, s);
-    convT_c1_out<TI, TO><<<nblk(u->d0.n()), 256, 0, s>>>(u->c1, u->d1, u->d0, layer_w(u, W, 7), r,
+    convT_c1_out_blk<TI, TO><<<nblk(u->d1.n()), 256, 0, s>>>(u->c1, u->d1, u->d0, layer_w(u, W, 7), r,
                                                          1.f, 1.0 / u->d0abs, f);
     SGPU_HIP(hipGetLastError());
     return SGPU_OK;
