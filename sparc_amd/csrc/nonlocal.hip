@@ -64,6 +64,18 @@ struct Mfma<double> {
     __device__ static int drow(int lk, int r) { return lk + 4 * r; }
 };
 
+// 16-byte vectors for staging copies
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    typedef float type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct Vec16<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+
 constexpr int kWaves = 4;
 constexpr int kRT = 4;            // 16-row tiles per projector chunk (64 rows)
 constexpr int kLDQ = kBP + 4;     // LDS stride of a staged column (bank-conflict free)
@@ -544,6 +556,17 @@ __global__ __launch_bounds__(256) void nls_apply(T *__restrict__ out, long ldo, 
     }
 }
 
+// Orders a wavefront's LDS writes before its later LDS reads and the reads
+// before later writes, for data exchanged between lanes of one wave only:
+// LDS instructions of a wave execute in order, so it is enough that the
+// compiler moves no LDS access across this point.
+__device__ __forceinline__ void wave_lds_order()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Chained applies (the Chebyshev recurrence, where every Hamiltonian output
 // is the next input), sphere layout with one colour (no two entries share a
 // grid point): after an entry's apply its points hold final values, so the
@@ -556,7 +579,13 @@ __global__ __launch_bounds__(256) void nls_apply(T *__restrict__ out, long ldo, 
 // Wave w takes, chunk by chunk, the quarter of the points that nls_project
 // gives it, in the same order, and the waves are summed in the same fixed
 // order, so the partials are bitwise those of the unfused path.
-template <typename T>
+// The waves run independently between the beta staging and the final sum:
+// a tile's grid indices are read two tiles ahead and its current output
+// values one tile ahead, and all of a tile's Chi operands are issued before
+// its first MFMA.  MRT: 16-row projector tiles held (npad <= 16 MRT).  CHL:
+// each kQC-point chunk of the entry's Chi is first copied to LDS by the
+// whole block (coalesced 16-byte loads) and the operands are read there.
+template <typename T, int MRT, bool CHL>
 __global__ __launch_bounds__(256) void nls_apply_proj(T *__restrict__ out, long ldo, int ncol,
                                                       int e0, const int *__restrict__ ent_info,
                                                       const long *__restrict__ ent_chi,
@@ -571,10 +600,12 @@ __global__ __launch_bounds__(256) void nls_apply_proj(T *__restrict__ out, long 
 {
     using M = Mfma<T>;
     using acc_t = typename M::acc;
-    constexpr int NC = M::NCA, NCT = NC / 16, LDB = NC * 3 / 2, LDY = NC + 1, MRT = 4;
-    __shared__ T beta[64 * LDB];
+    constexpr int NC = M::NCA, NCT = NC / 16, LDB = NC * 3 / 2, LDY = NC + 1;
+    constexpr int LDQ = kQC + 16;  // LDS stride of a staged Chi row
+    __shared__ T beta[16 * MRT * LDB];
     __shared__ T ytile[kWaves * 16 * LDY];
     __shared__ acc_t red[MRT * NCT][64];
+    __shared__ __align__(16) T chs[CHL ? 16 * MRT * LDQ : 1];
     const int lid = xcd_remap(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
     const int e = e0 + lid / gridDim.x;
     const int col0 = (lid % gridDim.x) * NC;
@@ -582,6 +613,32 @@ __global__ __launch_bounds__(256) void nls_apply_proj(T *__restrict__ out, long 
     const int np = ent_info[6 * e + 2], npad = ent_info[6 * e + 3], atom = ent_info[6 * e + 4];
     const int nrt = npad / 16;
     const int i0 = atom_img_ptr[atom], i1 = atom_img_ptr[atom + 1], row = atom_row[atom];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    // the wave's tiles: chunk t / 4 of kQC points, tile t % 4 of the wave's
+    // quarter of the chunk (every chunk but the last holds four per wave)
+    const int ntile = qpad / 64;
+    auto tile_q0 = [&](int t) {
+        const int qc = (t >> 2) * kQC;
+        return qc + wave * (min(kQC, qpad - qc) >> 2) + (t & 3) * 16;
+    };
+    const int *gp = gpos + poff;
+    auto load_out = [&](acc_t *ov, int g) {
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int col = col0 + ct * 16 + M::drow(lk, r);
+                const bool ok = g >= 0 && col < ncol;
+                const T v = out[ok ? (long)col * ldo + g : 0L];
+                ov[ct][r] = ok ? v : T(0);
+            }
+    };
+    // the first tile's outputs are requested before the beta staging
+    int g_cur = gp[tile_q0(0) + li];
+    int g_nxt = ntile > 1 ? gp[tile_q0(1) + li] : -1;
+    acc_t ov[NCT];
+    load_out(ov, g_cur);
     // beta = scale dV Gamma sum over the atom's images of part, four
     // elements per thread at a time with every load before the stores
     // (clamped: the tail repeats the last element)
@@ -615,74 +672,105 @@ __global__ __launch_bounds__(256) void nls_apply_proj(T *__restrict__ out, long 
     }
     __syncthreads();
     const T *ct_ = chit + ent_chi[e];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int li = lane & 15, lk = lane >> 4;
     T *yw = ytile + wave * 16 * LDY;
     acc_t pacc[MRT][NCT];
 #pragma unroll
     for (int rt = 0; rt < MRT; rt++)
 #pragma unroll
         for (int ct = 0; ct < NCT; ct++) pacc[rt][ct] = acc_t{0, 0, 0, 0};
-    for (int qc = 0; qc < qpad; qc += kQC) {
-        const int qn = min(kQC, qpad - qc);  // a multiple of 64: same tile count per wave
-        for (int tt = 0; tt < qn / 64; tt++) {
-            const int q0 = qc + wave * (qn / 4) + tt * 16;
-            const int g = gpos[poff + q0 + li];
-            acc_t acc[NCT], ov[NCT];
+    for (int t = 0; t < ntile; t++) {
+        const int q0 = tile_q0(t);
+        const int qc = (t >> 2) * kQC;
+        if (CHL && (t & 3) == 0) {
+            // stage Chi rows 0..npad of points qc .. qc + qn (all loads first)
+            using V = typename Vec16<T>::type;
+            constexpr int VW = sizeof(V) / sizeof(T), kS = 16 * MRT * kQC / VW / 256;
+            const int qv = min(kQC, qpad - qc) / VW, nv = npad * qv;
+            V v[kS];
 #pragma unroll
-            for (int ct = 0; ct < NCT; ct++) {
-                acc[ct] = acc_t{0, 0, 0, 0};
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int col = col0 + ct * 16 + M::drow(lk, r);
-                    ov[ct][r] = (g >= 0 && col < ncol) ? out[(long)col * ldo + g] : T(0);
-                }
+            for (int j = 0; j < kS; j++) {
+                const int f = min((int)threadIdx.x + 256 * j, nv - 1);
+                const int pr = f / qv, qq = (f - pr * qv) * VW;
+                v[j] = *reinterpret_cast<const V *>(ct_ + (long)pr * qpad + qc + qq);
             }
-            for (int jb = 0; jb < npad; jb += 16) {
-                T bqs[4];
+            __syncthreads();  // the previous chunk's operands are read
 #pragma unroll
-                for (int u = 0; u < 4; u++) bqs[u] = ct_[(long)(jb + 4 * u + lk) * qpad + q0 + li];
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int j = jb + 4 * u;
-#pragma unroll
-                    for (int ct = 0; ct < NCT; ct++)
-                        acc[ct] = M::run(beta[(j + lk) * LDB + ct * 16 + li], bqs[u], acc[ct]);
-                }
-            }
-#pragma unroll
-            for (int ct = 0; ct < NCT; ct++) {
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int cl = ct * 16 + M::drow(lk, r);
-                    const bool ok = g >= 0 && col0 + cl < ncol;
-                    const T y = ov[ct][r] + acc[ct][r];
-                    if (ok) out[(long)(col0 + cl) * ldo + g] = y;
-                    yw[li * LDY + cl] = ok ? y : T(0);
+            for (int j = 0; j < kS; j++) {
+                const int f = threadIdx.x + 256 * j;
+                if (f < nv) {
+                    const int pr = f / qv, qq = (f - pr * qv) * VW;
+                    *reinterpret_cast<V *>(chs + pr * LDQ + qq) = v[j];
                 }
             }
             __syncthreads();
-            T av[4][MRT];
+        }
+        // Chi operands of the tile: B (p, point) of the apply, A (p, point)
+        // of the projection (rows past npad are never read: nrt <= MRT)
+        T bq[MRT][4], av[4][MRT];
 #pragma unroll
-            for (int s4 = 0; s4 < 4; s4++)
+        for (int rt = 0; rt < MRT; rt++)
 #pragma unroll
-                for (int rt = 0; rt < MRT; rt++)
-                    av[s4][rt] = rt < nrt ? ct_[(long)(rt * 16 + li) * qpad + q0 + 4 * s4 + lk] : T(0);
+            for (int u = 0; u < 4; u++) {
+                const bool ok = rt < nrt;
+                if (CHL) {
+                    const int ql = q0 - qc;
+                    bq[rt][u] = ok ? chs[(rt * 16 + 4 * u + lk) * LDQ + ql + li] : T(0);
+                    av[u][rt] = ok ? chs[(rt * 16 + li) * LDQ + ql + 4 * u + lk] : T(0);
+                } else {
+                    bq[rt][u] = ok ? ct_[(long)(rt * 16 + 4 * u + lk) * qpad + q0 + li] : T(0);
+                    av[u][rt] = ok ? ct_[(long)(rt * 16 + li) * qpad + q0 + 4 * u + lk] : T(0);
+                }
+            }
+        // the next tile's outputs, and the grid index of the one after
+        acc_t ovn[NCT];
+        load_out(ovn, t + 1 < ntile ? g_nxt : -1);
+        const int g_nn = t + 2 < ntile ? gp[tile_q0(t + 2) + li] : -1;
+        acc_t acc[NCT];
 #pragma unroll
-            for (int s4 = 0; s4 < 4; s4++) {
-                T bv[NCT];
+        for (int ct = 0; ct < NCT; ct++) acc[ct] = acc_t{0, 0, 0, 0};
 #pragma unroll
-                for (int ct = 0; ct < NCT; ct++) bv[ct] = yw[(4 * s4 + lk) * LDY + ct * 16 + li];
+        for (int rt = 0; rt < MRT; rt++) {
+            if (rt < nrt) {
 #pragma unroll
-                for (int rt = 0; rt < MRT; rt++) {
-                    if (rt < nrt) {
+                for (int u = 0; u < 4; u++) {
+                    const int j = rt * 16 + 4 * u;
 #pragma unroll
-                        for (int ct = 0; ct < NCT; ct++)
-                            pacc[rt][ct] = M::run(av[s4][rt], bv[ct], pacc[rt][ct]);
-                    }
+                    for (int ct = 0; ct < NCT; ct++)
+                        acc[ct] = M::run(beta[(j + lk) * LDB + ct * 16 + li], bq[rt][u], acc[ct]);
                 }
             }
         }
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int cl = ct * 16 + M::drow(lk, r);
+                const bool ok = g_cur >= 0 && col0 + cl < ncol;
+                const T y = ov[ct][r] + acc[ct][r];
+                if (ok) out[(long)(col0 + cl) * ldo + g_cur] = y;
+                yw[li * LDY + cl] = ok ? y : T(0);
+            }
+        }
+        wave_lds_order();
+#pragma unroll
+        for (int s4 = 0; s4 < 4; s4++) {
+            T bv[NCT];
+#pragma unroll
+            for (int ct = 0; ct < NCT; ct++) bv[ct] = yw[(4 * s4 + lk) * LDY + ct * 16 + li];
+#pragma unroll
+            for (int rt = 0; rt < MRT; rt++) {
+                if (rt < nrt) {
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ct++)
+                        pacc[rt][ct] = M::run(av[s4][rt], bv[ct], pacc[rt][ct]);
+                }
+            }
+        }
+        wave_lds_order();
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++) ov[ct] = ovn[ct];
+        g_cur = g_nxt;
+        g_nxt = g_nn;
     }
     // fixed-order cross-wave sum: wave 3 -> 2 -> 1 -> 0
     for (int w = kWaves - 1; w >= 1; w--) {
@@ -826,9 +914,20 @@ int launch_vnl_chain_sphere(sgpu_context *ctx, int ncol, const T *x, long ldx, T
     }
     SGPU_CHECK(sgpu_tbegin(ctx, &t));
     if (want_next && nl.n_colors == 1) {
-        nls_apply_proj<T><<<dim3((ncol + NC - 1) / NC, nl.n_ent), 256, 0, ctx->stream>>>(
-            out, ldo, ncol, 0, nl.ent_info, nl.ent_chi, nl.gpos, chit, nl.atom_img_ptr,
-            nl.atom_img, nl.atom_row, nl.gamma, scale * nl.dV, part, max_np, part_next);
+        const dim3 grid((ncol + NC - 1) / NC, nl.n_ent);
+        auto go = [&](auto kern) {
+            kern<<<grid, 256, 0, ctx->stream>>>(out, ldo, ncol, 0, nl.ent_info, nl.ent_chi,
+                                                nl.gpos, chit, nl.atom_img_ptr, nl.atom_img,
+                                                nl.atom_row, nl.gamma, scale * nl.dV, part,
+                                                max_np, part_next);
+        };
+        // Chi staged in LDS for FP32 with up to 32 projector rows (53 KB of
+        // LDS per block, three blocks per CU); FP64 reads it from L2
+        constexpr bool F32 = sizeof(T) == 4;
+        if (nl.max_npad <= 32)
+            go(nls_apply_proj<T, 2, F32>);
+        else
+            go(nls_apply_proj<T, 4, false>);
         SGPU_HIP(hipGetLastError());
         nl.chain_cur ^= 1;
         *projected = true;
@@ -900,7 +999,7 @@ int build_sphere_layout(sgpu_context *ctx, int n_influence, const int *ndc, cons
     std::vector<double> chir, chit;
     std::vector<std::vector<int>> imgs(n_atoms);
     long coff = 0;
-    int max_qpad = 0;
+    int max_qpad = 0, max_npad = 0;
     for (int e = 0; e < n_influence; e++) {
         const int a = order[e];
         const int n = ndc[a], np = nproj[a];
@@ -915,6 +1014,7 @@ int build_sphere_layout(sgpu_context *ctx, int n_influence, const int *ndc, cons
         eoff[e] = coff;
         imgs[atom].push_back(e);
         max_qpad = std::max(max_qpad, qpad);
+        max_npad = std::max(max_npad, npad);
         for (int q = 0; q < qpad; q++) gp.push_back(q < n ? grid_pos[pos_off[a] + q] : -1);
         chir.resize(coff + (size_t)qpad * npad, 0.0);
         chit.resize(coff + (size_t)qpad * npad, 0.0);
@@ -941,6 +1041,7 @@ int build_sphere_layout(sgpu_context *ctx, int n_influence, const int *ndc, cons
     nl.n_atoms = n_atoms;
     nl.n_colors = n_colors;
     nl.max_qpad = max_qpad;
+    nl.max_npad = max_npad;
     nl.color_start = color_start;
     nl.dV = dV;
     SGPU_CHECK(upload(&nl.ent_info, info.data(), info.size(), s));

@@ -82,7 +82,7 @@ struct sgpu_nonlocal {
     // list and Chi, entries ordered by colour (no two entries of a colour
     // share a grid point), images of one atom found through atom_img
     int mode = 0;               // 0 brick layout, 1 sphere layout
-    int n_ent = 0, n_atoms = 0, n_colors = 0, max_qpad = 0;
+    int n_ent = 0, n_atoms = 0, n_colors = 0, max_qpad = 0, max_npad = 0;
     std::vector<int> color_start;   // host: first entry of each colour (+ end)
     int *ent_info = nullptr;    // per entry: qpad, pos offset, np, npad, atom, chi offset (6 ints)
     long *ent_chi = nullptr;    // per entry: element offset of its Chi blocks
