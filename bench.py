@@ -123,10 +123,11 @@ def cpu_baseline(wl, ncol_sample=48, degree=None):
                       f"vectors, degree {m}, {dt:.2f} s"}
 
 
-def scf_bench(name, unet_weights=None):
+def scf_bench(name, unet_weights=None, kerker_weights=None):
     """FP64 and mixed-precision device SCF of one psp8 system, plus the mixed
     run with the learned Poisson preconditioner when trained weights are
-    given and the grid allows it (dimensions multiples of 4, periodic)."""
+    given and the grid allows it (dimensions multiples of 4, periodic), and
+    the same with a second network as the Kerker solves' preconditioner."""
     from sparc_amd import gpu
     from sparc_amd.systems import scf_system
     t0 = time.perf_counter()
@@ -144,21 +145,28 @@ def scf_bench(name, unet_weights=None):
                        + ("Fermi-Dirac" if p["smear_type"] == 0 else "Gaussian")
                        + f" smearing {1.0 / p['beta']:.4g} Ha, tol_scf {p['tol_scf']:.3g}"}
     learned = unet_weights is not None and all(n % 4 == 0 for n in (g.nx, g.ny, g.nz))
-    plan = [("f64", gpu.F64, False), ("mixed", gpu.F32, False)]
+    plan = [("f64", gpu.F64, False, False), ("mixed", gpu.F32, False, False)]
     if learned:
-        plan.append(("mixed_learned", gpu.F32, True))
+        plan.append(("mixed_learned", gpu.F32, True, False))
+        if kerker_weights is not None:
+            plan.append(("mixed_learned_kerker", gpu.F32, True, True))
     runs = {}
     with gpu.Context(g) as ctx:
         ctx.set_nonlocal(st.nloc)
-        net = None
+        net = knet = None
         if learned:
             net = ctx.unet(seed=1)
             net.set_weights(unet_weights)
-        for tag, dt, use_net in plan:
+            if kerker_weights is not None:
+                knet = ctx.unet(seed=1)
+                knet.set_weights(kerker_weights)
+                knet.set_shift(-p["kerker_ktf"] ** 2)
+        for tag, dt, use_net, use_knet in plan:
             ctx.timer_reset()
             ctx.timer_enable(True)
             r = ctx.scf_run(st.b, st.rho_at, dict(p, dtype=dt), rho_core=st.rho_core,
-                            poisson_net=net if use_net else None)
+                            poisson_net=net if use_net else None,
+                            kerker_net=knet if use_knet else None)
             ctx.timer_enable(False)
             phases = {}
             for k in ("scf_lanczos", "scf_chefsi", "scf_poisson", "scf_kerker",
@@ -175,12 +183,14 @@ def scf_bench(name, unet_weights=None):
                         "poisson_iters": int(r["poisson_iters"]),
                         "poisson_fallbacks": int(r["poisson_fallbacks"]),
                         "kerker_iters": int(r["kerker_iters"]),
+                        "kerker_fallbacks": int(r["kerker_fallbacks"]),
                         "phase_ms_per_iter": phases}
             print(f"[bench] scf {name} {tag}: {r['scf_iters']} iters, "
                   f"{r['mean_iter_ms']:.2f} ms/iter, E/atom {r['Etot'] / st.n_atom:.10f}, "
                   f"Poisson AAR iterations {r['poisson_iters']}", flush=True)
-        if net is not None:
-            net.close()
+        for n_ in (net, knet):
+            if n_ is not None:
+                n_.close()
     f = runs["f64"]
     for tag in runs:
         if tag == "f64":
@@ -225,6 +235,46 @@ def train_unet(train_steps=400):
                           "(guess and after 1, 2, 4 SCF iterations), 52^3"}
         net.close()
     print(f"[bench] unet: {info['pairs']} pairs, harvest {th:.1f}s, train {tt:.1f}s, "
+          f"loss {info['loss_first']:.3e} -> {info['loss_last']:.3e}", flush=True)
+    return w, info
+
+
+def train_kerker_unet(train_steps=400):
+    """Train a second network for the Kerker solves -(Lap - kTF^2) Pf = Lf:
+    pairs from Jacobi-AAR solves (shift -kTF^2) whose right-hand sides are
+    (Lap - 0.1 kTF^2) applied to the density changes of the 64-atom Si SCF's
+    first iterations (the form of the mixing residuals Kerker_precond sees).
+    Returns the FP32 weights and a record of the training."""
+    from sparc_amd import gpu
+    from sparc_amd.systems import scf_system
+    st, p = scf_system("si64")
+    k2 = p["kerker_ktf"] ** 2
+    with gpu.Context(st.grid) as ctx:
+        ctx.set_nonlocal(st.nloc)
+        rhos = [st.rho_at]
+        for k in range(1, 6):
+            r = ctx.scf_run(st.b, st.rho_at, dict(p, max_scf=k, dtype=gpu.F32), rho_core=st.rho_core)
+            rhos.append(r["rho"])
+        net = ctx.unet(seed=1)
+        net.set_shift(-k2)
+        th = time.perf_counter()
+        for a, b in zip(rhos[:-1], rhos[1:]):
+            rhs = ctx.Lap_vec_mult(b - a, -k2 * p["kerker_thresh"]).ravel()
+            net.harvest(rhs, np.zeros(st.grid.nd), tol=1e-8, max_iter=3000, stride=2, max_pairs=60)
+        th = time.perf_counter() - th
+        tt = time.perf_counter()
+        hist = net.train(train_steps, batch=4, lr=2e-3, seed=5)
+        tt = time.perf_counter() - tt
+        w = net.get_weights()
+        info = {"pairs": net.npairs, "steps": train_steps, "batch": 4,
+                "harvest_s": round(th, 2), "train_s": round(tt, 2),
+                "loss_first": float(hist[:5].mean()), "loss_last": float(hist[-20:].mean()),
+                "shift": -k2,
+                "source": "Jacobi-AAR Kerker solves (tol 1e-8) with right-hand sides "
+                          "(Lap - 0.1 kTF^2) d_k, d_k the density changes of the si64 SCF's "
+                          "first five iterations, 52^3"}
+        net.close()
+    print(f"[bench] kerker unet: {info['pairs']} pairs, train {tt:.1f}s, "
           f"loss {info['loss_first']:.3e} -> {info['loss_last']:.3e}", flush=True)
     return w, info
 
@@ -317,7 +367,8 @@ def main(argv=None):
 
     if args.only == "scf":
         w = None if args.no_poisson else train_unet()[0]
-        print(json.dumps({name: scf_bench(name, w) for name in args.scf.split(",")}))
+        wk = None if args.no_poisson else train_kerker_unet()[0]
+        print(json.dumps({name: scf_bench(name, w, wk) for name in args.scf.split(",")}))
         return 0
     if args.only == "poisson":
         print(json.dumps(poisson_bench(*train_unet())))
@@ -398,7 +449,11 @@ def main(argv=None):
     trained = None if args.no_poisson else train_unet()
     if args.scf and args.scf != "none":
         w = trained[0] if trained else None
-        res["scf"] = {name: scf_bench(name, w) for name in args.scf.split(",")}
+        ktrained = None if args.no_poisson else train_kerker_unet()
+        wk = ktrained[0] if ktrained else None
+        if ktrained:
+            res["kerker_learned_precond"] = {"training": ktrained[1]}
+        res["scf"] = {name: scf_bench(name, w, wk) for name in args.scf.split(",")}
     if trained:
         res["poisson_learned_precond"] = poisson_bench(*trained)
     if not args.no_cpu_baseline:

@@ -218,6 +218,11 @@ typedef struct {
      * (poisson_precond_user = the sgpu_unet) gives the learned solve. */
     sgpu_precond_fn poisson_precond;
     void *poisson_precond_user;
+    /* precond_fun of the Kerker AAR solves (Kerker_precond,
+     * src/mixing.c:462-527, shift c = -kTF^2); NULL selects Jacobi.  A solve
+     * with it that does not converge is redone with Jacobi-AAR. */
+    sgpu_precond_fn kerker_precond;
+    void *kerker_precond_user;
 } sgpu_scf_params;
 
 typedef struct {
@@ -230,6 +235,7 @@ typedef struct {
     long poisson_iters, kerker_iters;  /* AAR iterations summed over the run */
     int poisson_fallbacks;  /* Poisson solves redone with Jacobi-AAR because the solve with
                                poisson_precond did not converge */
+    int kerker_fallbacks;   /* the same for the Kerker solves and kerker_precond */
 } sgpu_scf_result;
 
 /* Density-mixing SCF with CheFSI: replaces scf / scf_loop
@@ -276,6 +282,11 @@ int  sgpu_unet_loss_grad(sgpu_unet *net, const double *r, const double *e, int n
 int  sgpu_unet_harvest(sgpu_unet *net, const double *b, const double *x0, double tol,
                        int max_iter, int stride, int max_pairs, int *added);
 int  sgpu_unet_npairs(const sgpu_unet *net);
+/* The shift c (<= 0) of the operator -(Lap + c) that sgpu_unet_harvest solves
+ * and the training output scale 1/|D0 + c| use: 0 (the default) for the
+ * Poisson solve, -kTF^2 for the Kerker solve (Kerker_precond,
+ * src/mixing.c:462-527).  No counterpart in the reference. */
+int  sgpu_unet_set_shift(sgpu_unet *net, double c);
 /* Adam (lr, 0.9, 0.999) over the stored pairs, batch pairs per step. */
 int  sgpu_unet_train(sgpu_unet *net, int steps, int batch, double lr, unsigned long long seed,
                      double *loss_hist);

@@ -271,7 +271,7 @@ extern "C" int sgpu_scf_run(sgpu_context *ctx, const sgpu_scf_params *par, const
 
     // FP64 device vectors of stride ld, then the Pulay history and small arrays
     enum { RHO_IN, RHO_OUT, XKM1, FK, FKM1, PF, XW, FW, PHI, VXC, EXC, BCH, RHS, LF, ONES, CORE,
-           PHI0, NVEC };
+           PHI0, PF0, NVEC };
     double *mem = nullptr;
     const size_t elems = (size_t)(NVEC + 2 * m) * ld + ns + 64;
     SGPU_HIP(hipMalloc((void **)&mem, elems * sizeof(double)));
@@ -447,9 +447,24 @@ extern "C" int sgpu_scf_run(sgpu_context *ctx, const sgpu_scf_params *par, const
                                            true));
             int kit = 0;
             double krr = 0.0;
-            int kst = sgpu_aar_device(ctx, nullptr, nullptr, nullptr, -k2, (int)nd, vec[PF], vec[LF],
-                                      0.6, 0.6, 7, 6, par->tol_precond, 1000, &kit, &krr);
+            if (par->kerker_precond)
+                SGPU_HIP(hipMemcpyAsync(vec[PF0], vec[PF], nd * sizeof(double),
+                                        hipMemcpyDeviceToDevice, s));
+            int kst = sgpu_aar_device(ctx, nullptr, par->kerker_precond, par->kerker_precond_user,
+                                      -k2, (int)nd, vec[PF], vec[LF], 0.6, 0.6, 7, 6,
+                                      par->tol_precond, 1000, &kit, &krr);
             if (kst < 0) return kst;
+            if (kst != SGPU_OK && par->kerker_precond) {
+                // not converged with the caller's preconditioner: Jacobi-AAR
+                // (the reference's solve) from the same start
+                res->kerker_iters += kit;
+                res->kerker_fallbacks++;
+                SGPU_HIP(hipMemcpyAsync(vec[PF], vec[PF0], nd * sizeof(double),
+                                        hipMemcpyDeviceToDevice, s));
+                kst = sgpu_aar_device(ctx, nullptr, nullptr, nullptr, -k2, (int)nd, vec[PF],
+                                      vec[LF], 0.6, 0.6, 7, 6, par->tol_precond, 1000, &kit, &krr);
+                if (kst < 0) return kst;
+            }
             res->kerker_iters += kit;
             if (std::fabs(par->kerker_ktf) < 1e-14) {
                 double sum = 0.0;

@@ -732,7 +732,9 @@ void launch_convT_cc(const uint16_t *in, Dims di, uint16_t *out, Dims dout, cons
 struct sgpu_unet {
     sgpu_context *ctx = nullptr;
     Dims d0{}, d1{}, d2{};
+    double d0sum = -1.0;  // D2x[0] + D2y[0] + D2z[0] (the Laplacian's diagonal)
     double d0abs = 1.0;
+    double shift = 0.0;   // c of the operator Lap + c harvested for and trained on
     int nparam = 0;
     int off[kLayers + 1] = {0};
     float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr, *wt = nullptr;
@@ -765,9 +767,12 @@ int dalloc(sgpu_unet *u, T **p, size_t n)
 
 const float *layer_w(const sgpu_unet *u, const float *base, int l) { return base + u->off[l]; }
 
+// 1 / |D0 + shift|: the output scale of training and of sgpu_unet_apply
+double out_scale(const sgpu_unet *u) { return 1.0 / std::fabs(u->d0sum + u->shift); }
+
 // forward pass for one input (fine field r of type TI); writes f (type TO)
 template <typename TI, typename TO>
-int forward(sgpu_unet *u, const TI *r, TO *f)
+int forward(sgpu_unet *u, const TI *r, TO *f, double out_scale)
 {
     hipStream_t s = u->ctx->stream;
     const float *W = u->w;
@@ -782,7 +787,7 @@ int forward(sgpu_unet *u, const TI *r, TO *f)
     launch_convT_cc(u->c2, u->d2, u->s1, u->d1, layer_w(u, W, 5), wb(5), u->b1, s);
     launch_conv_cc<1>(u->s1, u->d1, u->c1, u->d1, layer_w(u, W, 6), wb(6), nullptr, s);
     convT_c1_out_blk<TI, TO><<<nblk(u->d1.n()), 256, 0, s>>>(u->c1, u->d1, u->d0, layer_w(u, W, 7), r,
-                                                         1.f, 1.0 / u->d0abs, f);
+                                                         1.f, out_scale, f);
     SGPU_HIP(hipGetLastError());
     return SGPU_OK;
 }
@@ -827,8 +832,7 @@ int backward(sgpu_unet *u, const float *r, const float *df)
         if (kLayerCin[l] == kC && kLayerCout[l] == kC)
             pack_bf16<<<nblk(kWcc), 256, 0, s>>>(wt, 1, u->wtb);
     };
-    // f = (r + U) / |D0|  ->  dU = df / |D0|: folded into the P1 passes as a scale
-    const float sU = (float)(1.0 / u->d0abs);
+    // f = (r + U) / |D0 + shift|  ->  dU = df / |D0 + shift|, applied by loss_grad
     float *dU = u->f32buf;  // overwrite df copy with dU
     // P1 (convT C->1, L1->L0): dW7[0][ci][k] = sum_y c1[y][ci] dU[2y+k-1]
     //   -> G[a=ci][b=0][k], transposed to [co=0][ci][k]
@@ -865,7 +869,6 @@ int backward(sgpu_unet *u, const float *r, const float *df)
     // R1 (conv/2 1->C, L0->L1): dW0[co][0][k] = sum_y da1[y][co] r[2y+k-1]
     SGPU_CHECK(wgrad_launch(u, 2, u->gb, u->d1, r, 1, u->d0, 0, 0));
     SGPU_HIP(hipGetLastError());
-    (void)sU;
     return SGPU_OK;
 }
 
@@ -879,10 +882,10 @@ int loss_grad_pairs(sgpu_unet *u, const float *const *R, const float *const *E, 
     const int lblocks = 256;
     std::vector<double> hp(lblocks);
     for (int i = 0; i < nb; i++) {
-        SGPU_CHECK((forward<float, float>(u, R[i], u->f32buf)));
-        // df = 2 (f - e) / nb, then dU = df / |D0|
-        loss_grad<<<lblocks, 256, 0, s>>>(u->f32buf, E[i], n0,
-                                          (float)(1.0 / (nb * u->d0abs)), u->f32buf, u->lpart);
+        SGPU_CHECK((forward<float, float>(u, R[i], u->f32buf, out_scale(u))));
+        // df = 2 (f - e) / nb, then dU = df / |D0 + shift|
+        loss_grad<<<lblocks, 256, 0, s>>>(u->f32buf, E[i], n0, (float)(out_scale(u) / nb),
+                                          u->f32buf, u->lpart);
         SGPU_HIP(hipGetLastError());
         SGPU_CHECK(backward(u, R[i], u->f32buf));
         SGPU_HIP(hipMemcpyAsync(hp.data(), u->lpart, lblocks * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -919,7 +922,8 @@ extern "C" int sgpu_unet_create(sgpu_context *ctx, int channels, unsigned long l
     u->d0 = Dims{ctx->nx, ctx->ny, ctx->nz};
     u->d1 = Dims{ctx->nx / 2, ctx->ny / 2, ctx->nz / 2};
     u->d2 = Dims{ctx->nx / 4, ctx->ny / 4, ctx->nz / 4};
-    u->d0abs = std::fabs(ctx->d2x[0] + ctx->d2y[0] + ctx->d2z[0]);
+    u->d0sum = ctx->d2x[0] + ctx->d2y[0] + ctx->d2z[0];
+    u->d0abs = std::fabs(u->d0sum);
     for (int l = 0; l < kLayers; l++) u->off[l + 1] = u->off[l] + layer_size(l);
     u->nparam = u->off[kLayers];
     auto fail = [&](int st) { sgpu_unet_destroy(u); return st; };
@@ -997,15 +1001,16 @@ extern "C" int sgpu_unet_set_weights(sgpu_unet *u, const float *w)
     return SGPU_OK;
 }
 
-// precond_fun of AAR (src/linearSolver.c:37-43): f = (r + U(r)) / |D0| on FP64
-// device vectors of the context's grid.  The shift c is not used (the Poisson
-// solve has c = 0; the Kerker solve keeps Jacobi).
+// precond_fun of AAR (src/linearSolver.c:37-43): f = (r + U(r)) / |D0 + c| on
+// FP64 device vectors of the context's grid, with |D0 + c| the magnitude of
+// the Jacobi diagonal of the operator Lap + c being solved (c = 0 for
+// Poisson, -kTF^2 for the Kerker solve).
 extern "C" void sgpu_unet_precond(void *user, int N, double c, const double *r_dev, double *f_dev)
 {
     sgpu_unet *u = static_cast<sgpu_unet *>(user);
-    (void)c;
     if (!u || N != u->d0.nx * u->d0.ny * u->d0.nz) return;
-    (void)forward<double, double>(u, r_dev, f_dev);
+    const double diag = std::fabs(u->d0sum + c);
+    (void)forward<double, double>(u, r_dev, f_dev, diag > 0.0 ? 1.0 / diag : 1.0 / u->d0abs);
 }
 
 extern "C" int sgpu_unet_apply(sgpu_unet *u, const double *r_host, double *f_host)
@@ -1018,7 +1023,7 @@ extern "C" int sgpu_unet_apply(sgpu_unet *u, const double *r_host, double *f_hos
     int st = SGPU_OK;
     if (hipMemcpyAsync(buf, r_host, n * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess)
         st = SGPU_ERR_HIP;
-    if (st == SGPU_OK) st = forward<double, double>(u, buf, buf + n);
+    if (st == SGPU_OK) st = forward<double, double>(u, buf, buf + n, out_scale(u));
     if (st == SGPU_OK &&
         (hipMemcpyAsync(f_host, buf + n, n * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
          hipStreamSynchronize(s) != hipSuccess))
@@ -1060,10 +1065,11 @@ extern "C" int sgpu_unet_loss_grad(sgpu_unet *u, const double *r_host, const dou
     return st;
 }
 
-// Jacobi-AAR solve of the context's Poisson problem (rhs b, start x0, the
-// reference's AAR parameters 0.6 / 0.6 / 7 / 6) that records every
-// `stride`-th iterate; afterwards each recorded (r_k, x_k) becomes the pair
-// (r_k / |e_k|, (e_k - mean e_k) / |e_k|) with e_k = x* - x_k.
+// Jacobi-AAR solve of -(Lap + shift) x = b (rhs b, start x0, the reference's
+// AAR parameters 0.6 / 0.6 / 7 / 6) that records every `stride`-th iterate;
+// afterwards each recorded (r_k, x_k) becomes the pair (r_k / |e_k|,
+// (e_k - mean e_k) / |e_k|) with e_k = x* - x_k (the mean is removed only for
+// shift 0, where constants are the operator's null space).
 extern "C" int sgpu_unet_harvest(sgpu_unet *u, const double *b_host, const double *x0_host,
                                  double tol, int max_iter, int stride, int max_pairs, int *added)
 {
@@ -1113,8 +1119,8 @@ extern "C" int sgpu_unet_harvest(sgpu_unet *u, const double *b_host, const doubl
     };
     int it = 0;
     double rr = 0.0;
-    int st = sgpu_aar_device(ctx, nullptr, nullptr, nullptr, 0.0, (int)n, x, b, 0.6, 0.6, 7, 6, tol,
-                             max_iter, &it, &rr, &hook);
+    int st = sgpu_aar_device(ctx, nullptr, nullptr, nullptr, u->shift, (int)n, x, b, 0.6, 0.6, 7, 6,
+                             tol, max_iter, &it, &rr, &hook);
     int made = 0;
     if (st >= 0) {
         // pairs from the recorded iterates (skipping any whose error vanished)
@@ -1136,7 +1142,7 @@ extern "C" int sgpu_unet_harvest(sgpu_unet *u, const double *b_host, const doubl
                 se += hp[i];
                 se2 += hp[nbk + i];
             }
-            const double mean = se / (double)n;
+            const double mean = u->shift == 0.0 ? se / (double)n : 0.0;
             const double en2 = se2 - (double)n * mean * mean;
             if (!(en2 > 0.0) || !std::isfinite(en2)) continue;
             make_pair<<<nblk(n), 256, 0, s>>>(rk, x, xk, mean, 1.0 / std::sqrt(en2), n,
@@ -1156,6 +1162,16 @@ extern "C" int sgpu_unet_harvest(sgpu_unet *u, const double *b_host, const doubl
 }
 
 extern "C" int sgpu_unet_npairs(const sgpu_unet *u) { return u ? u->npairs : 0; }
+
+extern "C" int sgpu_unet_set_shift(sgpu_unet *u, double c)
+{
+    if (!u || !(c <= 0.0) || !std::isfinite(c)) {
+        sgpu_set_error("sgpu_unet_set_shift: the shift must be finite and <= 0");
+        return SGPU_ERR_ARG;
+    }
+    u->shift = c;
+    return SGPU_OK;
+}
 
 // Adam over the stored pairs: `steps` steps of `batch` pairs drawn with a
 // splitmix64 stream from `seed`; loss_hist (may be NULL) gets each step's loss.

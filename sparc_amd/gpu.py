@@ -59,7 +59,8 @@ class SCFParams(C.Structure):
                 ("tol_precond", C.c_double), ("Esc", C.c_double), ("xc", C.c_int),
                 ("dtype", C.c_int),
                 ("psi_seed", C.c_ulonglong), ("lanczos_seed", C.c_ulonglong),
-                ("poisson_precond", C.c_void_p), ("poisson_precond_user", C.c_void_p)]
+                ("poisson_precond", C.c_void_p), ("poisson_precond_user", C.c_void_p),
+                ("kerker_precond", C.c_void_p), ("kerker_precond_user", C.c_void_p)]
 
 
 class SCFResult(C.Structure):
@@ -68,7 +69,8 @@ class SCFResult(C.Structure):
                 ("E2", C.c_double), ("E3", C.c_double), ("Entropy", C.c_double),
                 ("Efermi", C.c_double), ("scf_error", C.c_double), ("mean_iter_ms", C.c_double),
                 ("total_ms", C.c_double), ("poisson_iters", C.c_long),
-                ("kerker_iters", C.c_long), ("poisson_fallbacks", C.c_int)]
+                ("kerker_iters", C.c_long), ("poisson_fallbacks", C.c_int),
+                ("kerker_fallbacks", C.c_int)]
 
 
 RES_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_double, _dp, _dp, _dp)
@@ -127,6 +129,7 @@ _SIGS = {
     "sgpu_unet_apply": (C.c_int, [_vp, _dp, _dp]),
     "sgpu_unet_loss_grad": (C.c_int, [_vp, _dp, _dp, C.c_int, _dp, C.POINTER(C.c_float)]),
     "sgpu_unet_harvest": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_int, C.c_int, C.c_int, _ip]),
+    "sgpu_unet_set_shift": (C.c_int, [_vp, C.c_double]),
     "sgpu_unet_npairs": (C.c_int, [_vp]),
     "sgpu_unet_train": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, C.c_ulonglong, _dp]),
     "sgpu_timer_get": (C.c_int, [_vp, C.c_char_p, _dp, _ip]),
@@ -408,17 +411,23 @@ class Context:
         return lam
 
     # -- SCF
-    def scf_run(self, b, rho0, params: dict, rho_core=None, poisson_net: "UNet | None" = None):
+    def scf_run(self, b, rho0, params: dict, rho_core=None, poisson_net: "UNet | None" = None,
+                kerker_net: "UNet | None" = None):
         """Density-mixing SCF (sgpu_scf_run).  Returns a dict with the energies,
         eigenvalues, final density / potential and per-iteration history.
-        poisson_net: a trained U-Net of this context as the Poisson solves'
-        preconditioner (None: Jacobi, as the reference)."""
+        poisson_net / kerker_net: a trained U-Net of this context as the
+        preconditioner of the Poisson / Kerker AAR solves (None: Jacobi, as
+        the reference); one network may serve both."""
         p = SCFParams(**params)
-        if poisson_net is not None:
-            if poisson_net.ctx is not self:
+        pre = C.cast(self.lib.sgpu_unet_precond, C.c_void_p).value
+        for net, fn, user in ((poisson_net, "poisson_precond", "poisson_precond_user"),
+                              (kerker_net, "kerker_precond", "kerker_precond_user")):
+            if net is None:
+                continue
+            if net.ctx is not self:
                 raise ValueError("scf_run: the U-Net belongs to another context")
-            p.poisson_precond = C.cast(self.lib.sgpu_unet_precond, C.c_void_p).value
-            p.poisson_precond_user = poisson_net.h.value
+            setattr(p, fn, pre)
+            setattr(p, user, net.h.value)
         bb = _f64(b).ravel()
         r0 = _f64(rho0).ravel()
         rc = None if rho_core is None else _f64(rho_core).ravel()
@@ -518,6 +527,10 @@ class UNet:
         self.ctx._check(self.lib.sgpu_unet_loss_grad(self.h, _ptr(R), _ptr(E), npair, C.byref(loss),
                                                      g.ctypes.data_as(C.POINTER(C.c_float))))
         return loss.value, g
+
+    def set_shift(self, c: float):
+        """Operator -(Lap + c) of harvesting and training (0 Poisson, -kTF^2 Kerker)."""
+        self.ctx._check(self.lib.sgpu_unet_set_shift(self.h, c))
 
     def harvest(self, b, x0, tol=1e-8, max_iter=3000, stride=2, max_pairs=64) -> int:
         added = C.c_int()

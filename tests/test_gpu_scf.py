@@ -138,36 +138,55 @@ def si8u():
     FP64 Jacobi SCF, plus a U-Net trained on the device on (residual, error)
     pairs of Jacobi-AAR Poisson solves of the 64-atom cell (52^3: on 28^3 the
     quarter level is 7^3 and 400 steps do not train reliably), used across
-    grids as in the benchmark."""
+    grids as in the benchmark.  A second network is trained for the Kerker
+    solves (shift -kTF^2) on Jacobi-AAR solves whose right-hand sides are
+    (Lap - 0.1 kTF^2) applied to the 64-atom SCF's first density changes."""
     from sparc_amd import gpu
     from sparc_amd.systems import scf_system
     gpu.load()
     s64, p64 = scf_system("si64")
+    k2 = p64["kerker_ktf"] ** 2
     with gpu.Context(s64.grid) as ctx:
         ctx.set_nonlocal(s64.nloc)
-        r2 = ctx.scf_run(s64.b, s64.rho_at, dict(p64, max_scf=2, dtype=gpu.F32),
-                         rho_core=s64.rho_core)
+        rhos = [s64.rho_at]
+        for k in (1, 2, 3):
+            rhos.append(ctx.scf_run(s64.b, s64.rho_at, dict(p64, max_scf=k, dtype=gpu.F32),
+                                    rho_core=s64.rho_core)["rho"])
         net = ctx.unet(seed=1)
-        for rho in (s64.rho_at, r2["rho"]):
+        for rho in (rhos[0], rhos[2]):
             net.harvest(4.0 * np.pi * (s64.b + rho), np.zeros(s64.grid.nd), tol=1e-8,
                         max_iter=3000, stride=2, max_pairs=120)
         hist = net.train(400, batch=4, lr=2e-3, seed=5)
         w = net.get_weights()
         net.close()
+        knet = ctx.unet(seed=1)
+        knet.set_shift(-k2)
+        for a, b in zip(rhos[:-1], rhos[1:]):
+            knet.harvest(ctx.Lap_vec_mult(b - a, -k2 * p64["kerker_thresh"]).ravel(),
+                         np.zeros(s64.grid.nd), tol=1e-8, max_iter=3000, stride=2, max_pairs=60)
+        khist = knet.train(400, batch=4, lr=2e-3, seed=5)
+        wk = knet.get_weights()
+        knet.close()
     st, p = scf_system("si8u")
     with gpu.Context(st.grid) as ctx:
         ctx.set_nonlocal(st.nloc)
         rj = ctx.scf_run(st.b, st.rho_at, dict(p, dtype=gpu.F64), rho_core=st.rho_core)
-    return st, p, rj, w, hist
+    return st, p, rj, w, hist, wk, khist
 
 
-def _learned_scf(st, p, w):
+def _learned_scf(st, p, w, wk=None):
     from sparc_amd import gpu
     with gpu.Context(st.grid) as ctx:
         ctx.set_nonlocal(st.nloc)
         net = ctx.unet(seed=1)
         net.set_weights(w)
-        return ctx.scf_run(st.b, st.rho_at, p, rho_core=st.rho_core, poisson_net=net)
+        knet = None
+        if wk is not None:
+            knet = ctx.unet(seed=1)
+            knet.set_weights(wk)
+            knet.set_shift(-p["kerker_ktf"] ** 2)
+        return ctx.scf_run(st.b, st.rho_at, p, rho_core=st.rho_core, poisson_net=net,
+                           kerker_net=knet)
 
 
 def test_scf_learned_poisson_preconditioner(si8u):
@@ -176,7 +195,7 @@ def test_scf_learned_poisson_preconditioner(si8u):
     residual; tolerance 1e-6 Ha/atom stated here) with fewer Poisson AAR
     iterations in total and no solve handed back to Jacobi."""
     from sparc_amd import gpu
-    st, p, rj, w, hist = si8u
+    st, p, rj, w, hist, _, _ = si8u
     assert hist[-20:].mean() < 0.2 * hist[:5].mean()
     ru = _learned_scf(st, dict(p, dtype=gpu.F64), w)
     assert rj["converged"] and ru["converged"]
@@ -192,9 +211,49 @@ def test_scf_learned_preconditioner_falls_back_to_jacobi(si8u):
     maxit_poisson is redone with Jacobi-AAR from the same start, and the SCF
     reaches the Jacobi SCF's energy."""
     from sparc_amd import gpu
-    st, p, rj, w, _ = si8u
+    st, p, rj, w, _, _, _ = si8u
     bad = np.random.default_rng(3).standard_normal(w.size).astype(np.float32) * 0.5
     ru = _learned_scf(st, dict(p, dtype=gpu.F64, maxit_poisson=300), bad)
     assert ru["converged"]
     assert ru["poisson_fallbacks"] > 0
     assert abs(ru["Etot"] - rj["Etot"]) / st.n_atom < 1e-6
+
+
+def test_scf_learned_kerker_preconditioner(si8u):
+    """Kerker_precond (src/mixing.c:462-527) with the Kerker-trained U-Net as
+    precond_fun of its AAR solves: the SCF reaches the Jacobi SCF's energy
+    (1e-6 Ha/atom) with fewer Kerker AAR iterations in total and no solve
+    handed back to Jacobi."""
+    from sparc_amd import gpu
+    st, p, rj, w, _, wk, khist = si8u
+    assert khist[-20:].mean() < 0.2 * khist[:5].mean()
+    ru = _learned_scf(st, dict(p, dtype=gpu.F64), w, wk)
+    assert ru["converged"]
+    assert ru["kerker_fallbacks"] == 0
+    assert abs(ru["Etot"] - rj["Etot"]) / st.n_atom < 1e-6
+    assert ru["kerker_iters"] < rj["kerker_iters"], (ru["kerker_iters"], rj["kerker_iters"])
+
+
+def test_scf_learned_kerker_falls_back_to_jacobi(si8u):
+    """A Kerker preconditioner whose output is not finite (weights of 1e30
+    overflow FP32 within the network) stops each solve at its first residual
+    check; the solve is redone with Jacobi-AAR from the same start and the
+    SCF still reaches the Jacobi SCF's energy."""
+    from sparc_amd import gpu
+    st, p, rj, w, _, wk, _ = si8u
+    bad = np.full(wk.size, 1e30, np.float32)
+    ru = _learned_scf(st, dict(p, dtype=gpu.F64), w, bad)
+    assert ru["converged"]
+    assert ru["kerker_fallbacks"] > 0
+    assert abs(ru["Etot"] - rj["Etot"]) / st.n_atom < 1e-6
+
+
+def test_unet_shift_must_not_be_positive(si8u):
+    from sparc_amd import gpu
+    st = si8u[0]
+    with gpu.Context(st.grid) as ctx:
+        net = ctx.unet(seed=1)
+        with pytest.raises(gpu.SparcGPUError, match="shift"):
+            net.set_shift(0.5)
+        net.set_shift(-1.0)
+        net.close()
