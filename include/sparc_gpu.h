@@ -223,6 +223,12 @@ typedef struct {
      * with it that does not converge is redone with Jacobi-AAR. */
     sgpu_precond_fn kerker_precond;
     void *kerker_precond_user;
+    /* optional outputs for the forces after the SCF: the final orbitals
+     * (Xorb; a block of this context with nstates columns and the SCF's
+     * dtype) and Vxc of the final density (XCPotential, Nd host doubles);
+     * NULL skips either */
+    sgpu_block *psi_out;
+    double *vxc_out;
 } sgpu_scf_params;
 
 typedef struct {
@@ -237,6 +243,16 @@ typedef struct {
                                poisson_precond did not converge */
     int kerker_fallbacks;   /* the same for the Kerker solves and kerker_precond */
 } sgpu_scf_result;
+
+/* The integrals of the nonlocal forces for every orbital of psi:
+ * out[0] = dV <Chi, psi> (Compute_Integral_psi_Chi, src/forces.c:849-888) and
+ * out[1..3] = <Chi, d psi / dx>, <Chi, d psi / dy>, <Chi, d psi / dz>
+ * (Compute_Integral_Chi_Dpsi, :890-924, with the order-2n first-derivative
+ * stencil of Gradient_vectors_dir), periodic images summed into their atom's
+ * rows; out is [4][n_alpha][ncol] (row-major) host doubles.  The forces
+ * follow as -2 occfac sum_n g_n sum_p Gamma_p out[0] out[1+d]
+ * (Compute_force_nloc_by_integrals, :926-993). */
+int sgpu_nonlocal_integrals(sgpu_context *ctx, const sgpu_block *psi, double *out);
 
 /* Density-mixing SCF with CheFSI: replaces scf / scf_loop
  * (src/electronicGroundState.c:607-1147) for one spin-unpolarized Gamma-point

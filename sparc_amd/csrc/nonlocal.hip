@@ -237,7 +237,7 @@ __global__ void nlb_reduce(const T *__restrict__ part, int ncol, int n_alpha,
     const int col = (int)(t - (long)r * ncol);
     T s = T(0);
     for (int e = arow_ptr[r]; e < arow_ptr[r + 1]; e++) s += part[(long)arow_src[e] * ncol + col];
-    alpha_s[(long)r * ncol + col] = (T)(scale * gamma[r]) * s;
+    alpha_s[(long)r * ncol + col] = (T)(scale * (gamma ? gamma[r] : 1.0)) * s;
 }
 
 // out_b[q][col] += sum_j Phi_b[q][j] beta_b[j][col]  on 16x16x4 MFMA, computed
@@ -1073,6 +1073,177 @@ void sgpu_free_nonlocal(sgpu_nonlocal &nl)
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     nl = sgpu_nonlocal();
+}
+
+// ---- nonlocal force integrals -------------------------------------------
+namespace {
+
+// First derivative along one axis with the order-2F central stencil
+// (Gradient_vectors_dir, src/gradVecRoutines.c), periodic wrap-around or
+// zero outside a Dirichlet boundary; one thread per grid point and column.
+struct D1W {
+    double w[SGPU_MAX_FDN + 1];
+};
+
+template <typename T>
+__global__ void grad_axis(const T *__restrict__ x, long ldx, T *__restrict__ y, long ldy, int nx,
+                          int ny, int nz, int axis, int bc, int fdn, D1W w)
+{
+    const long nd = (long)nx * ny * nz;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nd) return;
+    const int col = blockIdx.y;
+    const int ix = (int)(i % nx), iy = (int)((i / nx) % ny), iz = (int)(i / ((long)nx * ny));
+    const int c = axis == 0 ? ix : axis == 1 ? iy : iz;
+    const int n = axis == 0 ? nx : axis == 1 ? ny : nz;
+    const long stride = axis == 0 ? 1L : axis == 1 ? (long)nx : (long)nx * ny;
+    const T *xc = x + (long)col * ldx + (i - (long)c * stride);
+    double s = 0.0;
+    for (int p = 1; p <= fdn; p++) {
+        int a = c + p, b = c - p;
+        double va = 0.0, vb = 0.0;
+        if (a >= n) a = bc ? -1 : a - n;
+        if (b < 0) b = bc ? -1 : b + n;
+        if (a >= 0) va = (double)xc[(long)a * stride];
+        if (b >= 0) vb = (double)xc[(long)b * stride];
+        s += w.w[p] * (va - vb);
+    }
+    y[(long)col * ldy + i] = (T)s;
+}
+
+// alpha[atom_row + p][col] = scale * sum over the atom's images of part
+// (sphere layout; images in the fixed order of atom_img)
+template <typename T>
+__global__ void nls_sum_images(const T *__restrict__ part, int ncol, int n_atoms, int max_np,
+                               const int *__restrict__ atom_img_ptr,
+                               const int *__restrict__ atom_img,
+                               const int *__restrict__ atom_row,
+                               const int *__restrict__ ent_info, double scale,
+                               T *__restrict__ alpha)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)n_atoms * max_np * ncol) return;
+    const int col = (int)(t % ncol);
+    const long ap = t / ncol;
+    const int p = (int)(ap % max_np), a = (int)(ap / max_np);
+    const int i0 = atom_img_ptr[a], i1 = atom_img_ptr[a + 1];
+    if (p >= ent_info[6 * atom_img[i0] + 2]) return;
+    double v = 0.0;
+    for (int i = i0; i < i1; i++) v += (double)part[((long)atom_img[i] * max_np + p) * ncol + col];
+    alpha[(long)(atom_row[a] + p) * ncol + col] = (T)(scale * v);
+}
+
+// alpha (n_alpha x ncol, device, type T) = scale * sum over images Chi' x
+template <typename T>
+int project_alpha(sgpu_context *ctx, int ncol, const T *x, long ldx, double scale, T *alpha)
+{
+    const sgpu_nonlocal &nl = ctx->nl;
+    constexpr int NCP = Mfma<T>::NCP;
+    void *buf = nullptr;
+    if (nl.mode == 1) {
+        const int max_np = 64;
+        SGPU_CHECK(sgpu_scratch(ctx, (size_t)nl.n_ent * max_np * ncol * sizeof(T) + 256, &buf));
+        T *part = (T *)buf;
+        nls_project<T><<<dim3((ncol + NCP - 1) / NCP, nl.n_ent), 256, 0, ctx->stream>>>(
+            x, ldx, ncol, nl.ent_info, nl.ent_chi, nl.gpos,
+            sizeof(T) == 8 ? (const T *)nl.chir64 : (const T *)nl.chir32, max_np, part);
+        SGPU_HIP(hipGetLastError());
+        const long total = (long)nl.n_atoms * max_np * ncol;
+        nls_sum_images<T><<<(unsigned)((total + 255) / 256), 256, 0, ctx->stream>>>(
+            part, ncol, nl.n_atoms, max_np, nl.atom_img_ptr, nl.atom_img, nl.atom_row, nl.ent_info,
+            scale, alpha);
+        SGPU_HIP(hipGetLastError());
+    } else {
+        if (nl.n_bricks > 65535) {
+            sgpu_set_error("nonlocal: %d bricks exceed the launch limit", nl.n_bricks);
+            return SGPU_ERR_ARG;
+        }
+        SGPU_CHECK(sgpu_scratch(ctx, (size_t)nl.n_rows * ncol * sizeof(T) + 256, &buf));
+        T *part = (T *)buf;
+        nlb_project<T><<<dim3((ncol + NCP - 1) / NCP, nl.n_bricks), 256, 0, ctx->stream>>>(
+            x, ldx, ncol, ctx->nx, ctx->ny, ctx->nz, nl.brick_org, nl.brick_np, nl.brick_row,
+            sizeof(T) == 8 ? (const T *)nl.phi64 : (const T *)nl.phi32, part);
+        SGPU_HIP(hipGetLastError());
+        const long total = (long)nl.n_alpha * ncol;
+        nlb_reduce<T><<<(unsigned)((total + 255) / 256), 256, 0, ctx->stream>>>(
+            part, ncol, nl.n_alpha, nl.arow_ptr, nl.arow_src, nullptr, scale, alpha);
+        SGPU_HIP(hipGetLastError());
+    }
+    return SGPU_OK;
+}
+
+template <typename T>
+int nonlocal_integrals_typed(sgpu_context *ctx, const sgpu_block *psi, double *out)
+{
+    const sgpu_nonlocal &nl = ctx->nl;
+    const int ncol = psi->ncol;
+    const long na = (long)nl.n_alpha * ncol;
+    hipStream_t s = ctx->stream;
+    T *grad = nullptr, *alpha = nullptr;
+    std::vector<T> host((size_t)na);
+    D1W w{};
+    int st = SGPU_OK;
+    auto fail = [&](int code) { st = code; };
+    if (hipMalloc((void **)&grad, (size_t)ncol * psi->ld * sizeof(T)) != hipSuccess ||
+        hipMalloc((void **)&alpha, (size_t)std::max(na, 1L) * sizeof(T)) != hipSuccess) {
+        sgpu_set_error("sgpu_nonlocal_integrals: device allocation failed");
+        fail(SGPU_ERR_HIP);
+    }
+    const double h[3] = {ctx->hx, ctx->hy, ctx->hz};
+    const int bc[3] = {ctx->bcx, ctx->bcy, ctx->bcz};
+    for (int k = 0; k < 4 && st == SGPU_OK; k++) {
+        const T *src = (const T *)psi->ptr;
+        if (k > 0) {
+            // w_p = (-1)^(p+1) F!^2 / ((F-p)! (F+p)!) / (p h)
+            //      (src/initialization.c:2113-2132)
+            for (int p = 1; p <= ctx->fdn; p++) {
+                double fr = 1.0;
+                for (int q = 1; q <= p; q++) fr *= (double)(ctx->fdn - p + q) / (double)(ctx->fdn + q);
+                w.w[p] = ((p % 2) ? 1.0 : -1.0) * fr / p / h[k - 1];
+            }
+            const dim3 grid((unsigned)((ctx->nd + 255) / 256), (unsigned)ncol);
+            grad_axis<T><<<grid, 256, 0, s>>>((const T *)psi->ptr, psi->ld, grad, psi->ld, ctx->nx,
+                                              ctx->ny, ctx->nz, k - 1, bc[k - 1], ctx->fdn, w);
+            if (hipGetLastError() != hipSuccess) {
+                sgpu_set_error("sgpu_nonlocal_integrals: gradient launch failed");
+                fail(SGPU_ERR_HIP);
+                break;
+            }
+            src = grad;
+        }
+        const int pst = project_alpha<T>(ctx, ncol, src, psi->ld, k == 0 ? nl.dV : 1.0, alpha);
+        if (pst < 0) {
+            fail(pst);
+            break;
+        }
+        if (hipMemcpyAsync(host.data(), alpha, (size_t)na * sizeof(T), hipMemcpyDeviceToHost, s) !=
+                hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+            sgpu_set_error("sgpu_nonlocal_integrals: copy failed");
+            fail(SGPU_ERR_HIP);
+            break;
+        }
+        for (long i = 0; i < na; i++) out[(long)k * na + i] = (double)host[i];
+    }
+    (void)hipStreamSynchronize(s);
+    if (grad) (void)hipFree(grad);
+    if (alpha) (void)hipFree(alpha);
+    return st;
+}
+
+}  // namespace
+
+// <Chi, psi> dV and <Chi, d psi / dx_d> for every orbital of psi (images
+// summed into the atom's rows): Compute_Integral_psi_Chi and
+// Compute_Integral_Chi_Dpsi (src/forces.c:849-924) with Gradient_vectors_dir;
+// out[4][n_alpha][ncol] on the host.
+extern "C" int sgpu_nonlocal_integrals(sgpu_context *ctx, const sgpu_block *psi, double *out)
+{
+    if (!ctx || !psi || !out || psi->ctx != ctx || !psi->ptr) return SGPU_ERR_ARG;
+    if (ctx->nl.n_alpha == 0 || psi->ncol <= 0) return SGPU_OK;
+    if (ctx->fdn < 1 || ctx->fdn > SGPU_MAX_FDN) return SGPU_ERR_ARG;
+    if (psi->dtype == SGPU_F64) return nonlocal_integrals_typed<double>(ctx, psi, out);
+    return nonlocal_integrals_typed<float>(ctx, psi, out);
 }
 
 extern "C" int sgpu_set_nonlocal(sgpu_context *ctx, int n_influence, const int *ndc,

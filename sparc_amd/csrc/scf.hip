@@ -262,6 +262,10 @@ extern "C" int sgpu_scf_run(sgpu_context *ctx, const sgpu_scf_params *par, const
         sgpu_set_error("sgpu_scf_run: bad parameters");
         return SGPU_ERR_ARG;
     }
+    if (par->psi_out && (par->psi_out->ctx != ctx || par->psi_out->ncol != ns)) {
+        sgpu_set_error("sgpu_scf_run: psi_out must be a block of this context with nstates columns");
+        return SGPU_ERR_ARG;
+    }
     SGPU_HIP(hipSetDevice(ctx->device));
     memset(res, 0, sizeof(*res));
     const long nd = ctx->nd, ld = ctx->ld;
@@ -513,6 +517,10 @@ extern "C" int sgpu_scf_run(sgpu_context *ctx, const sgpu_scf_params *par, const
             SGPU_HIP(hipMemcpyAsync(rho_out_host, rho_in, nd * sizeof(double), hipMemcpyDeviceToHost, s));
         if (phi_out_host)
             SGPU_HIP(hipMemcpyAsync(phi_out_host, phi, nd * sizeof(double), hipMemcpyDeviceToHost, s));
+        if (par->vxc_out)
+            SGPU_HIP(hipMemcpyAsync(par->vxc_out, vec[VXC], nd * sizeof(double), hipMemcpyDeviceToHost,
+                                    s));
+        if (par->psi_out) SGPU_CHECK(sgpu_block_copy(X, par->psi_out));
         SGPU_HIP(hipStreamSynchronize(s));
         return SGPU_OK;
     };

@@ -60,7 +60,8 @@ class SCFParams(C.Structure):
                 ("dtype", C.c_int),
                 ("psi_seed", C.c_ulonglong), ("lanczos_seed", C.c_ulonglong),
                 ("poisson_precond", C.c_void_p), ("poisson_precond_user", C.c_void_p),
-                ("kerker_precond", C.c_void_p), ("kerker_precond_user", C.c_void_p)]
+                ("kerker_precond", C.c_void_p), ("kerker_precond_user", C.c_void_p),
+                ("psi_out", C.c_void_p), ("vxc_out", C.c_void_p)]
 
 
 class SCFResult(C.Structure):
@@ -130,6 +131,7 @@ _SIGS = {
     "sgpu_unet_loss_grad": (C.c_int, [_vp, _dp, _dp, C.c_int, _dp, C.POINTER(C.c_float)]),
     "sgpu_unet_harvest": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_int, C.c_int, C.c_int, _ip]),
     "sgpu_unet_set_shift": (C.c_int, [_vp, C.c_double]),
+    "sgpu_nonlocal_integrals": (C.c_int, [_vp, _vp, _dp]),
     "sgpu_unet_npairs": (C.c_int, [_vp]),
     "sgpu_unet_train": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, C.c_ulonglong, _dp]),
     "sgpu_timer_get": (C.c_int, [_vp, C.c_char_p, _dp, _ip]),
@@ -232,6 +234,7 @@ class Context:
         self.h = h
         self._keep = []
         self._nets = weakref.WeakSet()  # U-Nets hold the context's stream: closed first
+        self._n_alpha = 0
 
     # -- plumbing
     def _check(self, st: int) -> int:
@@ -280,6 +283,7 @@ class Context:
         self._check(self.lib.sgpu_set_veff(self.h, _ptr(v)))
 
     def set_nonlocal(self, nl: NonlocalSet | None):
+        self._n_alpha = 0 if nl is None or nl.n_influence == 0 else int(nl.n_alpha)
         if nl is None or nl.n_influence == 0:
             z = np.zeros(1, np.int32)
             self._check(self.lib.sgpu_set_nonlocal(self.h, 0, None, None, None, None, None, None,
@@ -412,13 +416,21 @@ class Context:
 
     # -- SCF
     def scf_run(self, b, rho0, params: dict, rho_core=None, poisson_net: "UNet | None" = None,
-                kerker_net: "UNet | None" = None):
+                kerker_net: "UNet | None" = None, keep_psi: bool = False):
         """Density-mixing SCF (sgpu_scf_run).  Returns a dict with the energies,
         eigenvalues, final density / potential and per-iteration history.
         poisson_net / kerker_net: a trained U-Net of this context as the
         preconditioner of the Poisson / Kerker AAR solves (None: Jacobi, as
-        the reference); one network may serve both."""
+        the reference); one network may serve both.  keep_psi: also return
+        the final orbitals as a device Block ("psi") and Vxc ("vxc"), the
+        inputs of the forces."""
         p = SCFParams(**params)
+        psi = vxc = None
+        if keep_psi:
+            psi = self.block(p.nstates, p.dtype)
+            vxc = np.zeros(self.nd)
+            p.psi_out = psi.h.value
+            p.vxc_out = vxc.ctypes.data
         pre = C.cast(self.lib.sgpu_unet_precond, C.c_void_p).value
         for net, fn, user in ((poisson_net, "poisson_precond", "poisson_precond_user"),
                               (kerker_net, "kerker_precond", "kerker_precond_user")):
@@ -445,6 +457,17 @@ class Context:
         out = {f: getattr(res, f) for f, _ in SCFResult._fields_}
         out.update(lambda_=lam, rho=rho, phi=phi, iter_ms=ims[:k], iter_err=ierr[:k],
                    iter_etot=iet[:k], status=st)
+        if keep_psi:
+            out.update(psi=psi, vxc=vxc)
+        return out
+
+    def nonlocal_integrals(self, psi: "Block") -> np.ndarray:
+        """[4, n_alpha, ncol]: dV <Chi, psi> and <Chi, d psi / dx_d>
+        (sgpu_nonlocal_integrals)."""
+        n_alpha = self._n_alpha
+        out = np.zeros((4, n_alpha, psi.ncol))
+        if n_alpha:
+            self._check(self.lib.sgpu_nonlocal_integrals(self.h, psi.h, _ptr(out)))
         return out
 
     # -- learned preconditioner

@@ -1,6 +1,7 @@
 """Writes sic8_refout.json: the energies the reference's own test
 SiC_orthogonal_quick_relax (standard accuracy) prints for its first SCF
-(tests/SiC_orthogonal_quick_relax/standard/SiC_orthogonal_quick_relax.refout),
+(tests/SiC_orthogonal_quick_relax/standard/SiC_orthogonal_quick_relax.refout)
+and the atomic forces at that geometry (RELAXSTEP 1 of the .refgeopt),
 together with the input it ran (cell, grid, atoms, settings) and the
 effective input parameters it echoes; and copies the test's .inpt / .ion
 input files (data) to sic8/.
@@ -50,7 +51,14 @@ fix = {
     "entropy_term": grab("-Entropy*kb*T"),
     "fermi_level": grab("Fermi level"),
     "scf_history": hist,
+    "rms_force": grab("RMS force"),
+    "max_force": grab("Maximum force"),
 }
+geo = (case / "SiC_orthogonal_quick_relax.refgeopt").read_text().split(":RELAXSTEP: 2")[0]
+fblock = geo.split(":F(Ha/Bohr):")[1].split(":CELL:")[0]
+fix["forces_step1"] = [[float(v) for v in ln.split()] for ln in fblock.strip().splitlines()]
+rblock = geo.split(":R(Bohr):")[1].split(":F(Ha/Bohr):")[0]
+fix["positions_step1"] = [[float(v) for v in ln.split()] for ln in rblock.strip().splitlines()]
 # effective parameters echoed under "Input parameters" (the defaults it applied)
 head = first.split("Input parameters")[1].split("Cell")[0]
 fix["echo"] = {k: v.strip() for k, v in re.findall(r"^([A-Z_0-9]+):[ \t]*(.*)$", head, re.M)
