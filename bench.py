@@ -164,10 +164,30 @@ def scf_bench(name, unet_weights=None, kerker_weights=None):
         for tag, dt, use_net, use_knet in plan:
             ctx.timer_reset()
             ctx.timer_enable(True)
+            want_f = tag in ("f64", "mixed")
             r = ctx.scf_run(st.b, st.rho_at, dict(p, dtype=dt), rho_core=st.rho_core,
                             poisson_net=net if use_net else None,
-                            kerker_net=knet if use_knet else None)
+                            kerker_net=knet if use_knet else None, keep_psi=want_f)
             ctx.timer_enable(False)
+            frec = None
+            if want_f:
+                from sparc_amd import forces as F
+                from sparc_amd.sparc_io import occupations
+                occ = occupations(r["lambda_"], r["Efermi"], p["beta"], p["smear_type"])
+                t1 = time.perf_counter()
+                ints = ctx.nonlocal_integrals(r["psi"])
+                t_dev = time.perf_counter() - t1
+                fnl = F.nonlocal_forces_from_integrals(st, ints, occ, p["occfac"])
+                t1 = time.perf_counter()
+                floc = F.local_forces(st, r["phi"])
+                fxc = F.xc_forces(st, r["vxc"])
+                t_host = time.perf_counter() - t1
+                r["forces"] = F.symmetrize(floc + fnl + fxc)
+                r.pop("psi").free()
+                fn = np.linalg.norm(r["forces"], axis=1)
+                frec = {"nonlocal_integrals_ms": round(t_dev * 1e3, 2),
+                        "local_and_nlcc_host_ms": round(t_host * 1e3, 1),
+                        "max_force": float(fn.max()), "mean_force": float(fn.mean())}
             phases = {}
             for k in ("scf_lanczos", "scf_chefsi", "scf_poisson", "scf_kerker",
                       "scf_density_energy", "filter", "stencil", "nonlocal", "project_gemm",
@@ -185,6 +205,8 @@ def scf_bench(name, unet_weights=None, kerker_weights=None):
                         "kerker_iters": int(r["kerker_iters"]),
                         "kerker_fallbacks": int(r["kerker_fallbacks"]),
                         "phase_ms_per_iter": phases}
+            if frec:
+                out[tag]["forces"] = frec
             print(f"[bench] scf {name} {tag}: {r['scf_iters']} iters, "
                   f"{r['mean_iter_ms']:.2f} ms/iter, E/atom {r['Etot'] / st.n_atom:.10f}, "
                   f"Poisson AAR iterations {r['poisson_iters']}", flush=True)
@@ -200,6 +222,8 @@ def scf_bench(name, unet_weights=None, kerker_weights=None):
         out[f"dE_per_atom_mixed{sfx}_vs_f64"] = abs(m["Etot"] - f["Etot"]) / st.n_atom
         out[f"max_dlambda_mixed{sfx}_vs_f64"] = float(np.max(np.abs(m["lambda_"] - f["lambda_"])))
         out[f"speedup_mixed{sfx}_vs_f64"] = round(f["mean_iter_ms"] / m["mean_iter_ms"], 3)
+    if "forces" in runs.get("mixed", {}) and "forces" in f:
+        out["max_dforce_mixed_vs_f64"] = float(np.abs(runs["mixed"]["forces"] - f["forces"]).max())
     return out
 
 

@@ -43,7 +43,7 @@ CONST_KB = 3.1668115634556e-6    # Boltzmann constant in Ha/K (src/include/isddf
 # forces, stress or parallelisation only: read and ignored here
 _IGNORED = {
     "NP_SPIN_PARAL", "NP_KPOINT_PARAL", "NP_BAND_PARAL", "NP_DOMAIN_PARAL", "NP_DOMAIN_PHI_PARAL",
-    "EIG_SERIAL_MAXNS", "EIG_PARAL_BLKSZ", "EIG_PARAL_ORFAC", "EIG_PARAL_MAXNP", "PRINT_FORCES",
+    "EIG_SERIAL_MAXNS", "EIG_PARAL_BLKSZ", "EIG_PARAL_ORFAC", "EIG_PARAL_MAXNP",
     "PRINT_ATOMS", "PRINT_VELS", "PRINT_EIGEN", "PRINT_DENSITY", "PRINT_RESTART", "PRINT_RESTART_FQ",
     "PRINT_RELAXOUT", "PRINT_MDOUT", "PRINT_ORBITAL", "PRINT_ENERGY_DENSITY", "VERBOSITY",
     "OUTPUT_FILE", "CALC_STRESS", "CALC_PRES", "RELAX_FLAG", "RELAX_METHOD", "RELAX_NITER",
@@ -165,6 +165,8 @@ class Case:
     type_counts: list
     psp_files: list
     ignored: list = field(default_factory=list)
+    print_forces: bool = False  # PRINT_FORCES: forces after the SCF
+    ref_cutoff: float = 0.5     # REFERENCE_CUTOFF of the pseudocharge correction
 
 
 def _f(keys, k, default=None, idx=0):
@@ -325,7 +327,9 @@ def load_case(base, psp_dir=None) -> Case:
         "RHO_TRIGGER": str(params["rho_trigger"]),
     }
     st.meta.update(nelec=nelec)
-    return Case(base.name, g, st, params, echo, names, counts, files, ignored)
+    return Case(base.name, g, st, params, echo, names, counts, files, ignored,
+                print_forces=_i(keys, "PRINT_FORCES", 0) == 1,
+                ref_cutoff=_f(keys, "REFERENCE_CUTOFF", 0.5))
 
 
 _KNOWN = {
@@ -336,7 +340,7 @@ _KNOWN = {
     "TOL_PSEUDOCHARGE", "TOL_KERKER", "TOL_PRECOND", "PRECOND_KERKER_KTF", "PRECOND_KERKER_THRESH",
     "REFERENCE_CUTOFF", "MIXING_VARIABLE", "MIXING_PRECOND", "MIXING_HISTORY", "MIXING_PARAMETER",
     "MIXING_PARAMETER_SIMPLE", "PULAY_FREQUENCY", "PULAY_RESTART", "EXCHANGE_CORRELATION",
-    "MD_FLAG", "OFDFT_FLAG", "SQ_FLAG", "MLFF_FLAG", "D3_FLAG", "HUBBARD_FLAG",
+    "MD_FLAG", "OFDFT_FLAG", "SQ_FLAG", "MLFF_FLAG", "D3_FLAG", "HUBBARD_FLAG", "PRINT_FORCES",
 }
 
 
@@ -396,6 +400,29 @@ def write_out(path, case: Case, result: dict, title: str = "device SCF (MI355X)"
             ("Fermi level", result["Efermi"], "(Ha)")]
     for label, v, unit in rows:
         lines.append(f"{label:<35s}:{v:18.10E} {unit}")
+    if "forces" in result:
+        # printed as "RMS force": the mean of |F_J| (src/electronicGroundState.c:232-242)
+        fn = np.linalg.norm(np.asarray(result["forces"]), axis=1)
+        lines += [f"{'RMS force':<35s}:{fn.mean():18.10E} (Ha/Bohr)",
+                  f"{'Maximum force':<35s}:{fn.max():18.10E} (Ha/Bohr)"]
+    Path(path).write_text("\n".join(lines) + "\n")
+
+
+def write_static(path, case: Case, result: dict) -> None:
+    """The .static file: fractional atom positions per type, the total free
+    energy and the atomic forces, in the reference's layout (no stress)."""
+    g = case.grid
+    L = np.array([g.hx * g.nx, g.hy * g.ny, g.hz * g.nz])
+    bar = "*" * 75
+    lines = [bar, f"{'Atom positions':^75s}".rstrip(), bar]
+    for nm, t in zip(case.type_names, case.setup.types):
+        lines.append(f"Fractional coordinates of {nm}:")
+        for r in np.asarray(t.positions, float) / L:
+            lines.append("".join(f"{v:18.10f}" for v in r))
+    lines.append(f"Total free energy (Ha): {result['Etot']:.15E}")
+    if "forces" in result:
+        lines.append("Atomic forces (Ha/Bohr):")
+        lines += ["".join(f"{v:18.10E}" for v in row) for row in np.asarray(result["forces"])]
     Path(path).write_text("\n".join(lines) + "\n")
 
 
@@ -427,16 +454,27 @@ def read_out_energies(path) -> dict:
     return out
 
 
-def run_case(case: Case, dtype: int = 0, poisson_net=None, ctx=None) -> dict:
-    """The device SCF of a loaded case (one context on cuda:0 unless given)."""
+def run_case(case: Case, dtype: int = 0, poisson_net=None, ctx=None, forces=None) -> dict:
+    """The device SCF of a loaded case (one context on cuda:0 unless given),
+    followed by the atomic forces when forces is true (default: the case's
+    PRINT_FORCES): result["forces"] [n_atom, 3] Ha/Bohr and their parts in
+    result["forces_parts"]."""
     from . import gpu
+    from .forces import scf_forces
+    want = case.print_forces if forces is None else bool(forces)
     own = ctx is None
     if own:
         ctx = gpu.Context(case.grid)
     try:
         ctx.set_nonlocal(case.setup.nloc)
-        return ctx.scf_run(case.setup.b, case.setup.rho_at, dict(case.params, dtype=dtype),
-                           rho_core=case.setup.rho_core, poisson_net=poisson_net)
+        r = ctx.scf_run(case.setup.b, case.setup.rho_at, dict(case.params, dtype=dtype),
+                        rho_core=case.setup.rho_core, poisson_net=poisson_net, keep_psi=want)
+        if want:
+            f = scf_forces(ctx, case.setup, case.params, r, ref_cutoff=case.ref_cutoff)
+            r["forces"] = f["total"]
+            r["forces_parts"] = {k: v for k, v in f.items() if k != "total"}
+            r.pop("psi").free()
+        return r
     finally:
         if own:
             ctx.close()

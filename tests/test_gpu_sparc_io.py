@@ -3,7 +3,8 @@
 read back against the energies the reference printed for it
 (tests/golden/sic8_refout.json).  Tolerance: 1e-5 Ha/atom for the free
 energy (the accuracy gate; the hand-built SiC system measured 3.4e-9),
-1e-4 Ha for the Fermi level."""
+1e-4 Ha for the Fermi level.  The input asks for forces (PRINT_FORCES: 1):
+the .static forces read back within 5e-6 Ha/Bohr of the reference's."""
 import json
 from pathlib import Path
 
@@ -17,7 +18,8 @@ GOLDEN = Path(__file__).resolve().parent / "golden"
 
 def test_reference_input_files_run_on_the_device(tmp_path):
     from sparc_amd import gpu
-    from sparc_amd.sparc_io import load_case, read_out_energies, run_case, write_eigen, write_out
+    from sparc_amd.sparc_io import (load_case, read_out_energies, run_case, write_eigen, write_out,
+                                    write_static)
     gpu.load()
     ref = json.loads((GOLDEN / "sic8_refout.json").read_text())
     case = load_case(GOLDEN / "sic8" / "SiC_orthogonal_quick_relax")
@@ -33,3 +35,7 @@ def test_reference_input_files_run_on_the_device(tmp_path):
     occ = np.array([float(r.split()[2]) for r in
                     (tmp_path / "sic.eigen").read_text().splitlines()[5:]])
     assert occ.sum() == pytest.approx(case.params["nelec"], abs=1e-6)
+    write_static(tmp_path / "sic.static", case, res)
+    fl = (tmp_path / "sic.static").read_text().split("Atomic forces (Ha/Bohr):")[1].strip().splitlines()
+    f = np.array([[float(v) for v in ln.split()] for ln in fl])
+    assert np.abs(f - np.asarray(ref["forces_step1"])).max() < 5e-6

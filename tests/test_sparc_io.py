@@ -45,7 +45,8 @@ def test_reference_input_gives_the_parameters_the_reference_echoes(sic_case, ref
         a, b = float(sic_case.echo[k]), float(echo[k])
         assert abs(a - b) <= 5e-3 * abs(b), (k, a, b)
     assert sic_case.params["tol_precond"] == pytest.approx(0.34488 ** 2 * 1e-3, rel=1e-4)
-    assert set(sic_case.ignored) >= {"RELAX_FLAG", "CALC_STRESS", "PRINT_FORCES"}
+    assert set(sic_case.ignored) >= {"RELAX_FLAG", "CALC_STRESS"}
+    assert sic_case.print_forces and sic_case.ref_cutoff == 0.5
 
 
 def test_reference_input_builds_the_validated_sic_system(sic_case):
@@ -161,8 +162,20 @@ def test_out_and_eigen_files_read_back(tmp_path, sic_case, ref):
            "Etot": ref["total_free_energy"], "Eband": ref["band_structure_energy"],
            "Exc": ref["exchange_correlation_energy"], "Entropy": ref["entropy_term"],
            "Efermi": ef, "lambda_": lam}
+    res["forces"] = np.asarray(ref["forces_step1"])
     write_out(tmp_path / "s.out", sic_case, res)
     write_eigen(tmp_path / "s.eigen", sic_case, res)
+    from sparc_amd.sparc_io import write_static
+    write_static(tmp_path / "s.static", sic_case, res)
+    st_text = (tmp_path / "s.static").read_text()
+    fl = st_text.split("Atomic forces (Ha/Bohr):")[1].strip().splitlines()
+    assert np.allclose([[float(v) for v in ln.split()] for ln in fl], ref["forces_step1"], rtol=1e-9)
+    out_text = (tmp_path / "s.out").read_text()
+    import re
+    rms = float(re.search(r"RMS force\s*:\s*([-+0-9.E]+)", out_text).group(1))
+    mx = float(re.search(r"Maximum force\s*:\s*([-+0-9.E]+)", out_text).group(1))
+    assert rms == pytest.approx(ref["rms_force"], rel=1e-8)
+    assert mx == pytest.approx(ref["max_force"], rel=1e-8)
     back = read_out_energies(tmp_path / "s.out")
     assert back["Free energy per atom"] == pytest.approx(ref["free_energy_per_atom"], abs=1e-10)
     assert back["Total free energy"] == pytest.approx(ref["total_free_energy"], rel=1e-10)
