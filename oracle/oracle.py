@@ -354,6 +354,9 @@ def ref_lib():
         L.ref_lap_plus_diag.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                         C.c_int, _dp, _dp, _dp, C.c_int, C.c_double, C.c_double,
                                         C.c_double, _dp, _dp, C.c_int, _dp, C.c_int]
+        if hasattr(L, "ref_gradient"):
+            L.ref_gradient.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                       _dp, _dp, _dp, C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int]
         _ref = L
     return _ref
 
@@ -369,4 +372,22 @@ def ref_lap_plus_diag(grid, coeffs, x, a, b, c, v=None):
     L.ref_lap_plus_diag(grid.nx, grid.ny, grid.nz, grid.bcx, grid.bcy, grid.bcz, grid.order,
                         _p(d2x), _p(d2y), _p(d2z), x.shape[1], a, b, c, _p(vv), _p(x), x.shape[0],
                         _p(y), y.shape[0])
+    return y
+
+
+def ref_gradient(grid, x, axis):
+    """The reference's own Gradient_vec_dir (c = 0) along axis (0 x, 1 y,
+    2 z), or None where oracle/_ref is not built."""
+    L = ref_lib()
+    if L is None or not hasattr(L, "ref_gradient"):
+        return None
+    from .forces import d1_weights
+    w = [np.ascontiguousarray(d1_weights(grid.order, h)) for h in (grid.hx, grid.hy, grid.hz)]
+    x = np.asfortranarray(np.asarray(x, np.float64).reshape(grid.nd, -1))
+    y = np.zeros_like(x)
+    st = L.ref_gradient(grid.nx, grid.ny, grid.nz, grid.bcx, grid.bcy, grid.bcz, grid.order,
+                        _p(w[0]), _p(w[1]), _p(w[2]), x.shape[1], axis, _p(x), x.shape[0], _p(y),
+                        y.shape[0])
+    if st != 0:
+        raise RuntimeError("ref_gradient failed")
     return y

@@ -82,3 +82,46 @@ def test_oracle_scf_forces_match_reference_sic():
     # every term matters at this tolerance: dropping one moves some force by far more
     for part in (floc, fnl, fxc):
         assert np.abs(F.symmetrize(floc + fnl + fxc - part) - ref).max() > 1e-4
+
+
+@pytest.mark.parametrize("n,bc", [((13, 11, 9), (0, 0, 0)), ((16, 15, 14), (0, 1, 0)),
+                                  ((9, 12, 10), (1, 1, 1))])
+def test_oracle_gradient_matches_reference_gradient(n, bc):
+    """The oracle's first derivative (periodic axes wrapped; the device
+    kernel follows the same rule) equals SPARC's own compiled
+    Gradient_vec_dir (src/gradVecRoutines.c:61-316, oracle/_ref) bit for bit
+    on periodic grids, and on Dirichlet axes where the reference treats
+    values outside the domain as zero."""
+    from oracle import oracle as O
+    from oracle.forces import d1_weights
+    from sparc_amd.workloads import Grid
+    if O.ref_lib() is None:
+        pytest.skip("reference stencil (oracle/_ref) not built here")
+    g = Grid(*n, 0.4, 0.45, 0.5, *bc)
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal((g.nd, 3))
+    for axis in range(3):
+        ref = O.ref_gradient(g, x, axis)
+        h = (g.hx, g.hy, g.hz)[axis]
+        w = d1_weights(g.order, h)
+        V = x.T.reshape(3, g.nz, g.ny, g.nx)
+        ax = 3 - axis
+        nax = V.shape[ax]
+        out = np.zeros_like(V)
+        per = (g.bcx, g.bcy, g.bcz)[axis] == 0
+        for p in range(1, len(w)):
+            if per:
+                a, b = np.roll(V, -p, axis=ax), np.roll(V, p, axis=ax)
+            else:
+                a = np.zeros_like(V); b = np.zeros_like(V)
+                sl = [slice(None)] * 4
+                sl[ax] = slice(0, nax - p); dst = tuple(sl)
+                sl[ax] = slice(p, nax); src = tuple(sl)
+                a[dst] = V[src]
+                b[src] = V[dst]
+            out += w[p] * (a - b)
+        got = out.reshape(3, -1).T
+        if per:
+            from oracle.forces import gradient
+            assert np.array_equal(gradient(g, x, axis), ref)
+        assert np.array_equal(got, ref), axis
