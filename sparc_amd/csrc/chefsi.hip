@@ -416,13 +416,141 @@ int rotate_raw(sgpu_context *ctx, int dtype, int ns, const void *Y, long ldy, co
 // Same reduction as dsygvd, but rocSOLVER's dsygst took 16 ms of dsygvd's
 // 29 ms at n = 480 on MI355X where the two triangular solves take well under
 // one.
+// Householder reduction of a symmetric matrix to tridiagonal form inside one
+// workgroup, the matrix held in LDS: LAPACK dsytd2 with UPLO = 'L' (the
+// storage rocsolver_dormtr reads back): for i = 0..n-2, H(i) = I - tau v v'
+// with v(0:i) = 0, v(i+1) = 1 and v(i+2:n) stored in A(i+2:n, i); e(i) =
+// beta, d(i) = A(i,i).  Each step is dlarfg (beta = -sign(alpha) |(alpha, x)|,
+// tau = (beta - alpha) / beta, x /= alpha - beta), w = tau A22 v,
+// w -= (tau / 2)(w'v) v and the rank-2 update A22 -= v w' + w v' on the lower
+// triangle.  rocsolver's dsytrd takes ~1,800 dependent kernels for n = 480
+// (3.7 us each on the device); here every step is a few workgroup barriers.
+// The lower triangle is packed by columns: A(r, c), r >= c, at
+// c (2n - c + 1) / 2 + (r - c).  Sums use a fixed order (deterministic).
+constexpr int kTridMaxN = 180;   // packed triangle + work vectors fit in 160 KB of LDS
+constexpr int kTridThreads = 1024;
+
+__device__ __forceinline__ int tri_at(int n, int r, int c) { return c * (2 * n - c + 1) / 2 + (r - c); }
+
+// Sum over the workgroup in a fixed order (wave shuffles, then the 16 wave
+// sums in order); the total is valid in thread 0.
+__device__ __forceinline__ double tri_block_sum(double v, double *red)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < kTridThreads / 64; k++) s += red[k];
+    return s;
+}
+
+__global__ __launch_bounds__(kTridThreads) void sytd2_lower_lds(double *__restrict__ A, int lda,
+                                                                  int n, double *__restrict__ D,
+                                                                  double *__restrict__ E,
+                                                                  double *__restrict__ tau)
+{
+    extern __shared__ double sm[];
+    const int npk = n * (n + 1) / 2;
+    double *P = sm;                 // packed lower triangle
+    double *v = P + npk;            // Householder vector of the current step (n)
+    double *w = v + n;              // w (n)
+    double *part = w + n;           // partial sums (kTridThreads)
+    __shared__ double sc[4];        // tau, scale, beta, dot
+    const int t = threadIdx.x;
+    for (int e = t; e < n * n; e += kTridThreads) {
+        const int c = e / n, r = e - c * n;
+        if (r >= c) P[tri_at(n, r, c)] = A[(long)c * lda + r];
+    }
+    __syncthreads();
+    for (int i = 0; i + 1 < n; i++) {
+        const int m = n - i - 1;               // length of x = A(i+1:n, i)
+        const int col = tri_at(n, i + 1, i);   // packed index of A(i+1, i)
+        // xnorm^2 over A(i+2:n, i): per-thread partials, then thread 0 in order
+        double ss = 0.0;
+        for (int k = 1 + t; k < m; k += kTridThreads) ss += P[col + k] * P[col + k];
+        const double xn2 = tri_block_sum(ss, part);
+        if (t == 0) {
+            const double alpha = P[col];
+            if (xn2 == 0.0) {
+                sc[0] = 0.0; sc[1] = 1.0; sc[2] = alpha;
+            } else {
+                const double beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+                sc[0] = (beta - alpha) / beta;
+                sc[1] = 1.0 / (alpha - beta);
+                sc[2] = beta;
+            }
+        }
+        __syncthreads();
+        const double ti = sc[0];
+        for (int k = t; k < m; k += kTridThreads)
+            v[k] = k == 0 ? 1.0 : (ti != 0.0 ? P[col + k] * sc[1] : P[col + k]);
+        __syncthreads();
+        if (ti != 0.0)
+            for (int k = 1 + t; k < m; k += kTridThreads) P[col + k] = v[k];
+        if (ti != 0.0) {
+            // w = tau A22 v over the symmetric trailing block (rows split into J chunks)
+            const int J = max(1, kTridThreads / m);
+            {
+                const int r = t / J, j = t - r * J;
+                double s2 = 0.0;
+                if (r < m)
+                    for (int c = j; c < m; c += J) {
+                        const int rr = max(r, c), cc = min(r, c);
+                        s2 += P[tri_at(n, i + 1 + rr, i + 1 + cc)] * v[c];
+                    }
+                part[t] = s2;
+            }
+            __syncthreads();
+            if (t < m) {
+                double s3 = 0.0;
+                for (int j = 0; j < J; j++) s3 += part[t * J + j];
+                w[t] = ti * s3;
+            }
+            __syncthreads();
+            // alpha2 = -tau/2 (w'v), w += alpha2 v
+            double dd = 0.0;
+            for (int k = t; k < m; k += kTridThreads) dd += w[k] * v[k];
+            const double s4 = tri_block_sum(dd, part);
+            if (t == 0) sc[3] = -0.5 * ti * s4;
+            __syncthreads();
+            for (int k = t; k < m; k += kTridThreads) w[k] += sc[3] * v[k];
+            __syncthreads();
+            // A22 -= v w' + w v' (lower triangle)
+            for (int e = t; e < m * m; e += kTridThreads) {
+                const int c = e / m, r = e - c * m;
+                if (r >= c) P[tri_at(n, i + 1 + r, i + 1 + c)] -= v[r] * w[c] + w[r] * v[c];
+            }
+        }
+        __syncthreads();
+        if (t == 0) {
+            E[i] = sc[2];
+            D[i] = P[tri_at(n, i, i)];
+            tau[i] = ti;
+            P[col] = sc[2];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        D[n - 1] = P[tri_at(n, n - 1, n - 1)];
+        tau[n - 1] = 0.0;
+        E[n - 1] = 0.0;
+    }
+    __syncthreads();
+    for (int e = t; e < n * n; e += kTridThreads) {
+        const int c = e / n, r = e - c * n;
+        if (r >= c) A[(long)c * lda + r] = P[tri_at(n, r, c)];
+    }
+}
+
 int eigensolve_device(sgpu_context *ctx, int n, double *dA, double *dB, double *lambda)
 {
     void *buf = nullptr;
-    // D, E, info in the shared scratch (free again once lambda is on the host)
-    SGPU_CHECK(sgpu_scratch(ctx, 2 * (size_t)n * sizeof(double) + 64, &buf));
+    // D, E, tau, the tridiagonal eigenvectors and info in the shared scratch
+    // (free again once lambda is on the host)
+    SGPU_CHECK(sgpu_scratch(ctx, (3 * (size_t)n + (size_t)n * n) * sizeof(double) + 64, &buf));
     double *dD = (double *)buf, *dE = dD + n;
-    rocblas_int *dinfo = (rocblas_int *)(dE + n);
+    rocblas_int *dinfo = (rocblas_int *)(dE + 2 * n + (size_t)n * n);
     sgpu_tmark t;
     SGPU_CHECK(sgpu_tbegin(ctx, &t));
     rocblas_int info = 0;
@@ -442,11 +570,31 @@ int eigensolve_device(sgpu_context *ctx, int n, double *dA, double *dB, double *
     SGPU_BLAS(rocblas_dtrsm(ctx->blas, rocblas_side_right, rocblas_fill_upper,
                             rocblas_operation_none, rocblas_diagonal_non_unit, n, n, &one, dB, n,
                             dA, n));
-    SGPU_BLAS(rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, n, dA, n, dD,
-                               dE, dinfo));
-    SGPU_BLAS(rocblas_dtrsm(ctx->blas, rocblas_side_left, rocblas_fill_upper,
-                            rocblas_operation_none, rocblas_diagonal_non_unit, n, n, &one, dB, n,
-                            dA, n));
+    if (n <= kTridMaxN && ctx->lds_tridiag) {
+        // tridiagonal reduction in one workgroup, then the tridiagonal
+        // eigenvectors (dstedc) and the back-transformation Q Z (dormtr):
+        // the three steps of dsyevd with its dsytrd replaced
+        double *dtau = dE + n, *dC = dtau + n;
+        const size_t shm = ((size_t)n * (n + 1) / 2 + 2 * (size_t)n + kTridThreads) * sizeof(double);
+        SGPU_HIP(hipFuncSetAttribute((const void *)sytd2_lower_lds,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        sytd2_lower_lds<<<1, kTridThreads, shm, ctx->stream>>>(dA, n, n, dD, dE, dtau);
+        SGPU_HIP(hipGetLastError());
+        SGPU_BLAS(rocsolver_dstedc(ctx->blas, rocblas_evect_tridiagonal, n, dD, dE, dC, n, dinfo));
+        SGPU_BLAS(rocsolver_dormtr(ctx->blas, rocblas_side_left, rocblas_fill_lower,
+                                   rocblas_operation_none, n, n, dA, n, dtau, dC, n));
+        SGPU_BLAS(rocblas_dtrsm(ctx->blas, rocblas_side_left, rocblas_fill_upper,
+                                rocblas_operation_none, rocblas_diagonal_non_unit, n, n, &one, dB,
+                                n, dC, n));
+        SGPU_HIP(hipMemcpyAsync(dA, dC, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice,
+                                ctx->stream));
+    } else {
+        SGPU_BLAS(rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, n, dA, n,
+                                   dD, dE, dinfo));
+        SGPU_BLAS(rocblas_dtrsm(ctx->blas, rocblas_side_left, rocblas_fill_upper,
+                                rocblas_operation_none, rocblas_diagonal_non_unit, n, n, &one, dB,
+                                n, dA, n));
+    }
     SGPU_CHECK(sgpu_tend(ctx, &t, "eigensolve"));
     SGPU_HIP(hipMemcpyAsync(lambda, dD, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     SGPU_HIP(hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, ctx->stream));

@@ -129,6 +129,10 @@ struct sgpu_context {
     // fuse each filter step's nonlocal projection into the previous step's
     // apply (SPARC_AMD_NL_CHAIN=0 turns it off; results are bitwise equal)
     bool nl_chain = true;
+    // eigensolves of up to 180 states reduce to tridiagonal form in one
+    // workgroup's LDS instead of rocsolver's dsytrd (SPARC_AMD_LDS_TRIDIAG=0
+    // keeps dsyevd everywhere)
+    bool lds_tridiag = true;
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;
 

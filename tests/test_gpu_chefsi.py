@@ -169,6 +169,35 @@ def test_generalized_eigensolve_filtered_scale(gpu):
     assert np.max(np.abs(Q.T @ Mp @ Q - np.eye(n))) < 1e-9
 
 
+@pytest.mark.parametrize("n", [2, 24, 160, 180])
+def test_generalized_eigensolve_lds_tridiagonal(gpu, monkeypatch, n):
+    """Up to 180 states the reduction to tridiagonal form runs in one
+    workgroup's LDS (dsytd2 'L'), then dstedc + dormtr: eigenvalues against
+    LAPACK dsygvd and the rocsolver dsyevd path (SPARC_AMD_LDS_TRIDIAG=0) to
+    1e-10, Mp-orthonormal eigenvectors that diagonalize Hp, on the
+    filtered-block scale spread."""
+    from sparc_amd.workloads import Grid
+    rng = np.random.default_rng(70 + n)
+    d = 10.0 ** rng.uniform(-4, 4, n)
+    lam0 = np.sort(rng.uniform(-0.3, 0.5, n))
+    E = 1e-3 * rng.standard_normal((n, n))
+    Mp = np.sqrt(np.outer(d, d)) * (np.eye(n) + (E + E.T) / 2)
+    F = 1e-3 * rng.standard_normal((n, n))
+    Hp = np.sqrt(np.outer(d, d)) * (np.diag(lam0) + (F + F.T) / 2)
+    g = Grid(8, 8, 8, 0.5, 0.5, 0.5)
+    out = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("SPARC_AMD_LDS_TRIDIAG", flag)
+        with gpu.Context(g) as ctx:
+            out[flag] = ctx.DP_Solve_Generalized_EigenProblem(Hp, Mp)
+    lam, Q = out["1"]
+    lam_ref = sla.eigh(Hp, Mp, eigvals_only=True, driver="gvd")
+    assert np.max(np.abs(lam - lam_ref)) < 1e-10
+    assert np.max(np.abs(lam - out["0"][0])) < 1e-10
+    assert np.max(np.abs(Q.T @ Mp @ Q - np.eye(n))) < 1e-9
+    assert np.max(np.abs(Q.T @ Hp @ Q - np.diag(lam))) < 1e-9
+
+
 @pytest.mark.parametrize("ns", [24, 160, 300])
 def test_fp32_gram_accuracy(gpu, ns):
     """The FP32 projection (BF16-piece MFMA kernel below 256 states, rocBLAS
