@@ -254,6 +254,30 @@ typedef struct {
  * (Compute_force_nloc_by_integrals, :926-993). */
 int sgpu_nonlocal_integrals(sgpu_context *ctx, const sgpu_block *psi, double *out);
 
+/* Radial tables of one pseudopotential on its uniform grid r[0..size): r V_loc
+ * with its spline derivatives, the NLCC core density / 4 pi with its spline
+ * derivatives (NULL without NLCC), the valence charge and V_loc(0)
+ * (PSD_OBJ fields RadialGrid, rVloc, SplinerVlocD, rho_c_table, SplineRhocD,
+ * Znucl, Vloc_0). */
+typedef struct {
+    int size;
+    const double *r, *rvloc, *rvloc_d, *rhoc, *rhoc_d;
+    double zion, vloc0;
+} sgpu_psp_radial;
+
+/* Local and NLCC force terms (Calculate_local_forces_linear,
+ * src/forces.c:157-457; Calculate_forces_xc_linear, :473-707) for n_img
+ * image atoms: img_box[6k..6k+5] = xs, ys, zs, xe, ye, ze of image k's
+ * pseudocharge box (grid indices inside the grid), img_centre its position,
+ * img_type / img_atom its pseudopotential and atom; phi, Vc, b + b_ref and
+ * Vxc on the grid (host, Nd each).  f_loc / f_xc receive [n_atom][3]
+ * (not symmetrized).  Periodic orthogonal cells. */
+int sgpu_local_forces(sgpu_context *ctx, int n_types, const sgpu_psp_radial *types, int n_img,
+                      const int *img_type, const int *img_atom, const int *img_box,
+                      const double *img_centre, double ref_cutoff, const double *phi,
+                      const double *vc, const double *bsum, const double *vxc, int n_atom,
+                      double *f_loc, double *f_xc);
+
 /* Density-mixing SCF with CheFSI: replaces scf / scf_loop
  * (src/electronicGroundState.c:607-1147) for one spin-unpolarized Gamma-point
  * system whose grid, Veff storage and nonlocal projectors live in ctx.  b is

@@ -74,6 +74,11 @@ class SCFResult(C.Structure):
                 ("kerker_fallbacks", C.c_int)]
 
 
+class PspRadial(C.Structure):
+    _fields_ = [("size", C.c_int), ("r", _dp), ("rvloc", _dp), ("rvloc_d", _dp), ("rhoc", _dp),
+                ("rhoc_d", _dp), ("zion", C.c_double), ("vloc0", C.c_double)]
+
+
 RES_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_double, _dp, _dp, _dp)
 PRECOND_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_double, _dp, _dp)
 
@@ -132,6 +137,8 @@ _SIGS = {
     "sgpu_unet_harvest": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_int, C.c_int, C.c_int, _ip]),
     "sgpu_unet_set_shift": (C.c_int, [_vp, C.c_double]),
     "sgpu_nonlocal_integrals": (C.c_int, [_vp, _vp, _dp]),
+    "sgpu_local_forces": (C.c_int, [_vp, C.c_int, _vp, C.c_int, _ip, _ip, _ip, _dp, C.c_double, _dp,
+                                    _dp, _dp, _dp, C.c_int, _dp, _dp]),
     "sgpu_unet_npairs": (C.c_int, [_vp]),
     "sgpu_unet_train": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, C.c_ulonglong, _dp]),
     "sgpu_timer_get": (C.c_int, [_vp, C.c_char_p, _dp, _ip]),
@@ -460,6 +467,39 @@ class Context:
         if keep_psi:
             out.update(psi=psi, vxc=vxc)
         return out
+
+    def local_forces(self, setup, boxes, phi, vxc, ref_cutoff: float = 0.5):
+        """(F^loc, F^xc) [n_atom, 3] (sgpu_local_forces); boxes = (type,
+        atom, box, centre) arrays per image atom (sparc_amd.forces.image_boxes)."""
+        types, atoms, box, centre = boxes
+        keep = []
+        tabs = (PspRadial * len(setup.types))()
+        for i, t in enumerate(setup.types):
+            psp = t.psp
+            arrs = [np.ascontiguousarray(psp.r, np.float64), np.ascontiguousarray(psp.rVloc, np.float64),
+                    np.ascontiguousarray(psp.spline("rVloc"), np.float64)]
+            nlcc = bool(np.any(psp.rho_c_table))
+            if nlcc:
+                arrs += [np.ascontiguousarray(psp.rho_c_table, np.float64),
+                         np.ascontiguousarray(psp.spline("rho_c_table"), np.float64)]
+            keep += arrs
+            tabs[i].size = int(psp.r.size)
+            tabs[i].r, tabs[i].rvloc, tabs[i].rvloc_d = (_ptr(a) for a in arrs[:3])
+            tabs[i].rhoc = _ptr(arrs[3]) if nlcc else None
+            tabs[i].rhoc_d = _ptr(arrs[4]) if nlcc else None
+            tabs[i].zion = float(psp.zion)
+            tabs[i].vloc0 = float(psp.Vloc_0)
+        types = np.ascontiguousarray(types, np.int32); atoms = np.ascontiguousarray(atoms, np.int32)
+        box = np.ascontiguousarray(box, np.int32); centre = np.ascontiguousarray(centre, np.float64)
+        bsum = _f64(np.asarray(setup.b) + np.asarray(setup.b_ref)).ravel()
+        n_atom = int(setup.n_atom)
+        floc, fxc = np.zeros((n_atom, 3)), np.zeros((n_atom, 3))
+        self._check(self.lib.sgpu_local_forces(
+            self.h, len(setup.types), C.cast(tabs, C.c_void_p), int(types.size),
+            types.ctypes.data_as(_ip), atoms.ctypes.data_as(_ip), box.ctypes.data_as(_ip),
+            _ptr(centre), float(ref_cutoff), _ptr(_f64(phi).ravel()), _ptr(_f64(setup.Vc).ravel()),
+            _ptr(bsum), _ptr(_f64(vxc).ravel()), n_atom, _ptr(floc), _ptr(fxc)))
+        return floc, fxc
 
     def nonlocal_integrals(self, psi: "Block") -> np.ndarray:
         """[4, n_alpha, ncol]: dV <Chi, psi> and <Chi, d psi / dx_d>

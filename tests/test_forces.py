@@ -1,6 +1,6 @@
 """Atomic forces (SPARC Calculate_EGS_Forces, src/forces.c): the oracle SCF
-of the reference's own SiC test input followed by the force terms (host
-local / NLCC parts of sparc_amd/forces.py, the oracle's nonlocal integrals)
+of the reference's own SiC test input followed by the oracle's force terms
+(oracle/forces.py: local, NLCC and nonlocal)
 reproduces the forces the reference printed at that geometry
 (SiC_orthogonal_quick_relax.refgeopt, RELAXSTEP 1); the first-derivative
 weights are checked against the derivative of a plane wave."""
@@ -19,8 +19,7 @@ def test_first_derivative_weights_against_plane_wave(order):
     """sum_p w_p (e^{ikph} - e^{-ikph}) = i k + O((kh)^(order+1)): at kh =
     0.3 the order-2n stencil's error falls below (kh)^(2n+1) / h, and both
     the host weights and the oracle's agree."""
-    from oracle.forces import d1_weights
-    from sparc_amd.forces import fd_d1
+    from oracle.forces import d1_weights, fd_d1
     h = 0.4
     w = fd_d1(order, h)
     assert np.allclose(w, d1_weights(order, h), rtol=1e-15, atol=0)
@@ -30,7 +29,7 @@ def test_first_derivative_weights_against_plane_wave(order):
         assert abs(sym - k) < (kh ** (order + 1)) / h + 1e-13
     # a plane wave on a periodic grid: the stencil maps sin(kx) to
     # (sum_p 2 w_p sin(p k h)) cos(kx) exactly, along each axis
-    from sparc_amd.forces import grad_periodic
+    from oracle.forces import grad_periodic
     n = 32
     x = np.arange(n) * h
     k = 2 * math.pi * 3 / (n * h)
@@ -50,6 +49,7 @@ def test_oracle_scf_forces_match_reference_sic():
     (its SCF stopped at error 8.6e-7; this SCF reproduces the same energies
     to 1e-7 Ha/atom), and the printed "RMS" (mean |F_J|) and maximum force within 1e-6."""
     from oracle import oracle as orc
+    from oracle import forces as O
     from oracle.forces import nonlocal_forces
     from oracle.scf import OracleSCF
     from sparc_amd import forces as F
@@ -68,8 +68,8 @@ def test_oracle_scf_forces_match_reference_sic():
     r = s.run()
     assert r["converged"]
     occ = occupations(r["lambda_"], r["Efermi"], p["beta"], p["smear_type"])
-    floc = F.local_forces(st, s.phi)
-    fxc = F.xc_forces(st, s.vxc)
+    floc = O.local_forces(st, s.phi)
+    fxc = O.xc_forces(st, s.vxc)
     fnl = nonlocal_forces(st.grid, st.nloc, F.atom_rows(st), st.n_atom, r["X"], occ, p["occfac"])
     f = F.symmetrize(floc + fnl + fxc)
     ref = np.asarray(gold["forces_step1"])

@@ -59,6 +59,29 @@ def test_nonlocal_integrals_psp_sic(gpu):
         assert rel_err(got[k], ref[k]) < 1e-12, k
 
 
+@pytest.mark.parametrize("name", ["sic8", "si8"])
+def test_local_forces_match_oracle(gpu, name):
+    """sgpu_local_forces (local + NLCC terms) against the oracle's box loop on
+    the same fields (smooth random phi / Vxc), SiC (two types, boxes reaching
+    across every face) and Si: 1e-11 Ha/Bohr."""
+    from oracle import forces as O
+    from sparc_amd import forces as F
+    from sparc_amd.systems import sic8, scf_system
+    st = sic8()[0] if name == "sic8" else scf_system("si8")[0]
+    g = st.grid
+    rng = np.random.default_rng(8)
+    k = [2 * np.pi * np.arange(n) / n for n in (g.nx, g.ny, g.nz)]
+    Z, Y, X = np.meshgrid(k[2], k[1], k[0], indexing="ij")
+    phi = (np.sin(X + 0.3) * np.cos(2 * Y) + 0.5 * np.sin(Z - Y)).ravel() + 0.01 * rng.standard_normal(g.nd)
+    vxc = (-0.3 + 0.1 * np.cos(X) * np.sin(Y + Z)).ravel()
+    with gpu.Context(g) as ctx:
+        floc, fxc = F.local_forces(ctx, st, phi, vxc)
+    ref_loc, ref_xc = O.local_forces(st, phi), O.xc_forces(st, vxc)
+    assert np.abs(floc - ref_loc).max() < 1e-11, np.abs(floc - ref_loc).max()
+    assert np.abs(fxc - ref_xc).max() < 1e-11, np.abs(fxc - ref_xc).max()
+    assert np.abs(ref_loc).max() > 1e-3 and np.abs(ref_xc).max() > 1e-5
+
+
 @pytest.mark.parametrize("dtype,tol", [(0, 5e-6), (1, 2e-5)])
 def test_scf_forces_match_reference_sic(gpu, dtype, tol):
     """Device SCF (FP64, and mixed precision) of the SiC input, then the
