@@ -445,6 +445,10 @@ void launch_cfg(sgpu_context *ctx, const StencilParams<T> &p, int ncol, int cfg)
         case 8: launch_tiled<T, 64, 16, 2>(ctx, p, ncol); break;
         case 9: launch_tiled<T, 32, 16, 2>(ctx, p, ncol, 8); break;
         case 10: launch_tiled<T, 32, 16, 2>(ctx, p, ncol, 4); break;
+        // whole x-rows for grids of 65-96 / 65-80 points along x: the x halo
+        // is the block's own row (L2 hits), so only y halos cost HBM traffic
+        case 11: launch_tiled<T, 96, 16, 2>(ctx, p, ncol); break;
+        case 12: launch_tiled<T, 80, 16, 2>(ctx, p, ncol); break;
         default: launch_tiled<T, 32, 16, 2>(ctx, p, ncol); break;
     }
 }
@@ -461,8 +465,10 @@ int tune_tile(sgpu_context *ctx, StencilParams<T> p, int ncol, int *best_cfg)
     // few columns (the Poisson / Kerker residual, Lanczos): also thinner z-chunks
     static const int cands_many[] = {0, 7, 8}, cands_few[] = {0, 7, 8, 9, 10};
     const bool few = ncol < 16;
-    const int *cands = few ? cands_few : cands_many;
-    const int ncand = few ? 5 : 3;
+    int cands[8], ncand = 0;
+    for (int i = 0; i < (few ? 5 : 3); i++) cands[ncand++] = few ? cands_few[i] : cands_many[i];
+    if (ctx->nx > 64 && ctx->nx <= 80) cands[ncand++] = 12;
+    else if (ctx->nx > 80 && ctx->nx <= 96) cands[ncand++] = 11;
     SGPU_HIP(hipStreamSynchronize(ctx->stream));
     T *tmp = nullptr;
     SGPU_HIP(hipMalloc((void **)&tmp, (size_t)ncol * p.ldo * sizeof(T)));
@@ -586,10 +592,11 @@ extern "C" int sgpu_stencil_tile(const sgpu_context *ctx, int dtype, int ncol, i
         }
         cfg = it->second;
     }
-    static const int shape[11][3] = {{32, 16, 2}, {32, 32, 4}, {64, 16, 4}, {32, 32, 8},
+    static const int shape[13][3] = {{32, 16, 2}, {32, 32, 4}, {64, 16, 4}, {32, 32, 8},
                                      {64, 32, 8}, {32, 16, 2}, {32, 8, 2},  {32, 32, 2},
-                                     {64, 16, 2}, {32, 16, 2}, {32, 16, 2}};
-    if (cfg > 10 || ((cfg == 5 || cfg == 6) && dtype == SGPU_F64)) cfg = 0;
+                                     {64, 16, 2}, {32, 16, 2}, {32, 16, 2}, {96, 16, 2},
+                                     {80, 16, 2}};
+    if (cfg > 12 || ((cfg == 5 || cfg == 6) && dtype == SGPU_F64)) cfg = 0;
     *tx = shape[cfg][0];
     *ty = shape[cfg][1];
     *ry = shape[cfg][2];

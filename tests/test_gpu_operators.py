@@ -35,7 +35,7 @@ def test_lap_plus_diag_fp64(gpu, oracle_mod, n, bc):
             assert rel_err(y, yo) < 1e-13, (a, b, c)
 
 
-@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4, 7, 8, 9, 10])
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4, 7, 8, 9, 10, 11, 12])
 @pytest.mark.parametrize("n,bc", [((40, 35, 13), (0, 0, 0)), ((70, 33, 9), (0, 1, 1)),
                                   ((7, 6, 30), (0, 0, 0))])
 def test_stencil_tile_shapes(gpu, oracle_mod, monkeypatch, cfg, n, bc):
