@@ -431,7 +431,7 @@ def main(argv=None):
     print(f"[bench] fp32 filter {ms32:.2f} ms ({gbs32:.0f} GB/s), stencil launch {st_ms:.3f} ms "
           f"({st_gbs:.0f} GB/s), nonlocal {r32['nonlocal'][0] / max(r32['nonlocal'][1], 1):.3f} ms",
           flush=True)
-    kernel = "stencil_tiled<float,%d,%d,%d>" % ctx.stencil_tile(gpu.F32, ns)
+    kernel = ctx.stencil_kernel(gpu.F32, ns)
     res = {
         "metric": METRIC,
         "value": round(gbs32, 2),

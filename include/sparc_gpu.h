@@ -66,6 +66,10 @@ int sgpu_sync(sgpu_context *ctx);
  * ry rows per thread) the radius-6 stencil uses for blocks of ncol columns of
  * precision dtype; 0 with *tx = 0 if no launch of that shape has chosen one. */
 int sgpu_stencil_tile(const sgpu_context *ctx, int dtype, int ncol, int *tx, int *ty, int *ry);
+/* Diagnostic, no counterpart in the reference: the name of that stencil
+ * kernel ("stencil_tiled<float,32,16,2>", "stencil_rows<96,16,2>", ...) into
+ * name[len]; an empty string if no launch of that shape has chosen one. */
+int sgpu_stencil_kernel(const sgpu_context *ctx, int dtype, int ncol, char *name, int len);
 
 /* Veff_loc_dmcomm (one spin channel, Nd values).  Replaces the field read by
  * Hamiltonian_vectors_mult (src/hamiltonianVecRoutines.c:45-49). */

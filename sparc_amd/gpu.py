@@ -89,6 +89,7 @@ _SIGS = {
     "sgpu_last_error": (C.c_char_p, []),
     "sgpu_get_stencil_coeffs": (C.c_int, [_vp, _dp, _dp, _dp]),
     "sgpu_stencil_tile": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _ip, _ip]),
+    "sgpu_stencil_kernel": (C.c_int, [_vp, C.c_int, C.c_int, C.c_char_p, C.c_int]),
     "sgpu_sync": (C.c_int, [_vp]),
     "sgpu_set_veff": (C.c_int, [_vp, _dp]),
     "sgpu_set_nonlocal": (C.c_int, [_vp, C.c_int, _ip, _ip, _lp, _ip, _lp, _dp, _ip, C.c_int, _dp,
@@ -283,6 +284,14 @@ class Context:
         t = [C.c_int() for _ in range(3)]
         self._check(self.lib.sgpu_stencil_tile(self.h, dtype, ncol, *[C.byref(v) for v in t]))
         return None if t[0].value == 0 else tuple(v.value for v in t)
+
+    def stencil_kernel(self, dtype: int, ncol: int):
+        """Name of the stencil kernel launched for ncol columns of dtype (as
+        the profiler lists it, without spaces), or None before a launch of
+        that shape has chosen one."""
+        buf = C.create_string_buffer(96)
+        self._check(self.lib.sgpu_stencil_kernel(self.h, dtype, ncol, buf, 96))
+        return buf.value.decode() or None
 
     def set_veff(self, veff: np.ndarray):
         v = _f64(veff).ravel()

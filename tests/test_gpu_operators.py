@@ -49,6 +49,35 @@ def test_stencil_tile_shapes(gpu, oracle_mod, monkeypatch, cfg, n, bc):
     assert rel_err(y, o.lap_plus_diag(x, -0.5, 1.0, 0.4, veff)) < 1e-13
 
 
+@pytest.mark.parametrize("cfg,n,bc", [(13, (96, 21, 40), (0, 0, 0)), (13, (96, 21, 10), (1, 0, 1)),
+                                      (14, (80, 19, 33), (0, 1, 0)), (14, (80, 37, 17), (1, 1, 0))])
+def test_stencil_whole_row_tiles(gpu, oracle_mod, monkeypatch, cfg, n, bc):
+    """Whole-row FP32 tiles (x halo from the block's own rows, y halo queued
+    kR planes ahead; z-chunks of 20 / 17 planes at nz 40 / 33): bitwise the
+    32x16x2 tile, and the FP32 result against the FP64 oracle."""
+    g, veff, nl, rng = small_case(n=n, bc=bc, seed=70 + cfg)
+    o = oracle_mod.Oracle(g, veff)
+    x = rng.uniform(-0.5, 0.5, (g.nd, 2))
+    z = rng.uniform(-0.5, 0.5, (g.nd, 2))
+    outs = {}
+    for c in (cfg, 0):
+        monkeypatch.setenv("SPARC_AMD_STENCIL_TILE", str(c))
+        with gpu.Context(g) as ctx:
+            ctx.set_veff(veff)
+            bx = ctx.block(2, gpu.F32).upload(x)
+            bz = ctx.block(2, gpu.F32).upload(z)
+            ctx.chebyshev_filter_dev(bx, bz, 2, 2.0, 9.0, -1.0)
+            outs[c] = (bx.download(), bz.download())
+            name = ctx.stencil_kernel(gpu.F32, 2)
+        assert name == ("stencil_rows<%d,16,2>" % n[0] if c == cfg else "stencil_tiled<float,32,16,2>")
+    assert np.array_equal(outs[cfg][0], outs[0][0]) and np.array_equal(outs[cfg][1], outs[0][1])
+    monkeypatch.setenv("SPARC_AMD_STENCIL_TILE", str(cfg))
+    with gpu.Context(g) as ctx:
+        y = ctx.Lap_plus_diag_vec_mult(x, -0.5, 1.0, 0.4, veff)
+        assert ctx.stencil_kernel(gpu.F64, 2) == "stencil_tiled<double,%d,16,2>" % n[0]
+    assert rel_err(y, o.lap_plus_diag(x, -0.5, 1.0, 0.4, veff)) < 1e-13
+
+
 @pytest.mark.parametrize("dtype", [0, 1])
 def test_stencil_tile_tuning_is_bitwise_neutral(gpu, monkeypatch, dtype):
     """The default tile choice (timed once per shape) gives bitwise the output
