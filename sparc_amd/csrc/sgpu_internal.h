@@ -120,9 +120,11 @@ struct sgpu_context {
     // 1 = 32x32x4, 2 = 64x16x4, 3 = 32x32x8, 4 = 64x32x8, 5 / 6 = 32x16x2 /
     // 32x8x2 with an 8-plane LDS ring (FP32; FP64 uses 0), 7 = 32x32x2,
     // 8 = 64x16x2 (512 threads), 9 / 10 = 32x16x2 with z-chunks of at least
-    // 8 / 4 planes (default 16); -1 (default) = the fastest of 0 / 7 / 8
-    // (and 9 / 10 below 16 columns), timed once per (precision, column
-    // count) and kept in tile_choice.  SPARC_AMD_STENCIL_TILE fixes one at
+    // 8 / 4 planes (default 16), 11 / 12 = 96x16x2 / 80x16x2 (whole x-rows
+    // on grids of 81-96 / 65-80 points), 13 / 14 = the same shapes as
+    // stencil_rows when nx is exactly 96 / 80 (FP32); -1 (default) = the
+    // fastest of 0 / 7 / 8 (9 / 10 below 16 columns, 11-14 where they fit),
+    // timed once per (precision, column count) and kept in tile_choice.  SPARC_AMD_STENCIL_TILE fixes one at
     // creation.
     int stencil_cfg = -1;
     std::map<std::pair<int, int>, int> tile_choice;
