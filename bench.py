@@ -203,7 +203,8 @@ def scf_bench(name, unet_weights=None, kerker_weights=None):
                         "poisson_fallbacks": int(r["poisson_fallbacks"]),
                         "kerker_iters": int(r["kerker_iters"]),
                         "kerker_fallbacks": int(r["kerker_fallbacks"]),
-                        "phase_ms_per_iter": phases}
+                        "phase_ms_per_iter": phases,
+                        "stencil_kernel": ctx.stencil_kernel(dt, p["nstates"])}
             if frec:
                 out[tag]["forces"] = frec
             print(f"[bench] scf {name} {tag}: {r['scf_iters']} iters, "

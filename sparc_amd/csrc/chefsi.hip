@@ -284,13 +284,17 @@ rocblas_status gemm_tn_batched<float>(rocblas_handle h, int n, long k, const flo
                                          (rocblas_int)ldb, sb, &zero, C, n, sc, batch);
 }
 
-bool gram_use_rocblas()
+// SPARC_AMD_GRAM=rocblas: rocBLAS for every FP32 Gram; =bf16x3: the BF16-piece
+// kernel for every FP32 Gram; unset: BF16 pieces below 256 columns.
+int gram_mode()
 {
     static const int v = [] {
         const char *e = getenv("SPARC_AMD_GRAM");
-        return (e && !strcmp(e, "rocblas")) ? 1 : 0;
+        if (e && !strcmp(e, "rocblas")) return 1;
+        if (e && !strcmp(e, "bf16x3")) return 2;
+        return 0;
     }();
-    return v != 0;
+    return v;
 }
 
 template <typename T>
@@ -303,10 +307,12 @@ int gram_tn_typed(sgpu_context *ctx, int n, long nd, const T *A, long lda, const
     void *buf = nullptr;
     SGPU_CHECK(sgpu_scratch(ctx, (size_t)nslice * nn * sizeof(T), &buf));
     T *part = (T *)buf;
-    // BF16-piece kernel for Ns < 256; above that its 64 x 64 output tiles
-    // re-read Y about 16 times and rocBLAS's FP32 GEMM was measured faster
-    // (474k x 480: 5.2 vs 11.9 ms per SCF iteration with the 2-piece variant)
-    if (sizeof(T) == 4 && n < 256 && !gram_use_rocblas()) {
+    // BF16-piece kernel for Ns < 256; above that rocBLAS's FP32 GEMM was
+    // measured faster (512k x 480, 216-atom SCF: 6.1 vs 19.5 ms per iteration;
+    // the per-block split of every staged value into three pieces, not the
+    // re-reads of Y, bounds it: mapping a K-slice's tiles onto one XCD gave
+    // the same time)
+    if (sizeof(T) == 4 && ((n < 256 && gram_mode() == 0) || gram_mode() == 2)) {
         const int tilesn = (n + kGT - 1) / kGT;
         gram_bf16x3<<<dim3(tilesn * tilesn, nslice), 256, 0, ctx->stream>>>(
             (const float *)A, lda, (const float *)B, ldb, n, nd, kslice, (float *)part);
