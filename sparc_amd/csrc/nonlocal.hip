@@ -1451,8 +1451,11 @@ int sgpu_launch_vnl_chain(sgpu_context *ctx, int dtype, int ncol, const void *x,
     // brick layout: a fused apply + projection kept the projection's
     // accumulators for up to 128 projector rows live across the brick and ran
     // at 2 waves per SIMD, slower (3.25 ms per application on the 96^3 x 512
-    // operator case) than the two kernels (2.55 ms); only the sphere layout
-    // carries partials
+    // operator case) than the two kernels (2.55 ms).  A second form (the
+    // apply for 16 columns writing the brick's final values to an LDS slice,
+    // then nlb_project's product on it) was also slower: 2.69 against
+    // 2.37 ms, since nlb_project is bound by its Phi loads and MFMA chain, not
+    // by staging X.  Only the sphere layout carries partials.
     if (nl.mode != 1) return sgpu_launch_vnl(ctx, dtype, ncol, x, ldx, out, ldo, scale);
     const int st =
         dtype == SGPU_F64
