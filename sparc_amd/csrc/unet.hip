@@ -523,40 +523,65 @@ __global__ __launch_bounds__(256) void convT_c1_out_blk(const uint16_t *__restri
 // Weight gradient G[a][b][k] = sum_y A[y][a] B[S y + k - 1][b] over the
 // coarse voxels y of this block's chunk (partial per block; summed in a
 // fixed order by wgrad_reduce).  A: BF16 channels-last (16 channels); B:
-// BF16 channels-last (CB = 16) or a scalar FP32 field (CB = 1).
+// BF16 channels-last (CB = 16) or a scalar FP32 field (CB = 1).  The chunk's
+// A rows are staged in LDS once; each thread's B values are gathered kWB
+// voxels at a time, all loads before the FMAs (one dependent load per voxel
+// left the kernel latency-bound at ~190 us per 26^3 layer).  Every (a, b, k)
+// still accumulates over y in order, so the sums are unchanged.
+constexpr int kWChunk = 32;   // coarse voxels per block (26^3: 550 blocks)
+constexpr int kWB = 16;       // B gathers in flight per thread
 template <int S, int CB>
 __global__ __launch_bounds__(256) void wgrad(const uint16_t *__restrict__ A, Dims da,
-                                             const void *__restrict__ Bv, Dims db, int chunk,
+                                             const void *__restrict__ Bv, Dims db,
                                              float *__restrict__ partial)
 {
     constexpr int NPK = 27 * CB;                      // (k, b) pairs
     constexpr int PPT = (NPK + 255) / 256;            // pairs per thread
+    __shared__ float as[kWChunk][kC];
     float acc[PPT][kC];
 #pragma unroll
     for (int p = 0; p < PPT; p++)
 #pragma unroll
         for (int a = 0; a < kC; a++) acc[p][a] = 0.f;
-    const long y0 = (long)blockIdx.x * chunk;
-    const long y1 = min(y0 + chunk, da.n());
-    for (long y = y0; y < y1; y++) {
-        const int cx = (int)(y % da.nx), cy = (int)((y / da.nx) % da.ny), cz = (int)(y / ((long)da.nx * da.ny));
-        float av[kC];
-        load16(A + y * kC, av);
+    const long y0 = (long)blockIdx.x * kWChunk;
+    const int ny = (int)min((long)kWChunk, da.n() - y0);
+    for (int e = threadIdx.x; e < ny * kC; e += 256) as[e / kC][e % kC] = bf2f(A[y0 * kC + e]);
+    __syncthreads();
+    int pk_k[PPT], pk_b[PPT];
 #pragma unroll
-        for (int p = 0; p < PPT; p++) {
-            const int pk = threadIdx.x + p * 256;
-            if (pk >= NPK) continue;
-            const int k = pk / CB, b = pk % CB;
-            const int ix = wrap(S * cx + k % 3 - 1, db.nx);
-            const int iy = wrap(S * cy + (k / 3) % 3 - 1, db.ny);
-            const int iz = wrap(S * cz + k / 9 - 1, db.nz);
-            float bv;
-            if (CB == 1)
-                bv = bfround(reinterpret_cast<const float *>(Bv)[db.vox(ix, iy, iz)]);
-            else
-                bv = bf2f(reinterpret_cast<const uint16_t *>(Bv)[db.vox(ix, iy, iz) * kC + b]);
+    for (int p = 0; p < PPT; p++) {
+        const int pk = min((int)threadIdx.x + p * 256, NPK - 1);
+        pk_k[p] = pk / CB;
+        pk_b[p] = pk % CB;
+    }
+    for (int yb = 0; yb < ny; yb += kWB) {
+        float bv[kWB][PPT];
 #pragma unroll
-            for (int a = 0; a < kC; a++) acc[p][a] += av[a] * bv;
+        for (int u = 0; u < kWB; u++) {
+            const long y = y0 + min(yb + u, ny - 1);  // tail: a valid voxel, not accumulated
+            const int cx = (int)(y % da.nx), cy = (int)((y / da.nx) % da.ny),
+                      cz = (int)(y / ((long)da.nx * da.ny));
+#pragma unroll
+            for (int p = 0; p < PPT; p++) {
+                const int k = pk_k[p], b = pk_b[p];
+                const int ix = wrap(S * cx + k % 3 - 1, db.nx);
+                const int iy = wrap(S * cy + (k / 3) % 3 - 1, db.ny);
+                const int iz = wrap(S * cz + k / 9 - 1, db.nz);
+                if (CB == 1)
+                    bv[u][p] = bfround(reinterpret_cast<const float *>(Bv)[db.vox(ix, iy, iz)]);
+                else
+                    bv[u][p] = bf2f(reinterpret_cast<const uint16_t *>(Bv)[db.vox(ix, iy, iz) * kC + b]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kWB; u++) {
+            if (yb + u >= ny) break;
+#pragma unroll
+            for (int p = 0; p < PPT; p++) {
+                if ((int)threadIdx.x + p * 256 >= NPK) continue;
+#pragma unroll
+                for (int a = 0; a < kC; a++) acc[p][a] += as[yb + u][a] * bv[u][p];
+            }
         }
     }
     // partial[block][a][b][k]
@@ -579,8 +604,19 @@ __global__ void wgrad_reduce(const float *__restrict__ partial, int nblocks, int
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = na * nb * 27;
     if (i >= n) return;
-    float s = 0.f;
-    for (int b = 0; b < nblocks; b++) s += partial[(long)b * n + i];
+    // eight interleaved running sums (blocks b = u mod 8), combined pairwise:
+    // eight loads in flight per thread instead of one
+    float su[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int b = 0;
+    for (; b + 8 <= nblocks; b += 8) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = partial[(long)(b + u) * n + i];
+#pragma unroll
+        for (int u = 0; u < 8; u++) su[u] += v[u];
+    }
+    for (int u = 0; b < nblocks; b++, u++) su[u] += partial[(long)b * n + i];
+    const float s = ((su[0] + su[1]) + (su[2] + su[3])) + ((su[4] + su[5]) + (su[6] + su[7]));
     const int k = i % 27, bb = (i / 27) % nb, a = i / (27 * nb);
     const int o = transpose ? (bb * na + a) * 27 + k : i;
     dW[o] += s;
@@ -796,19 +832,18 @@ int wgrad_launch(sgpu_unet *u, int stride, const uint16_t *A, Dims da, const voi
                  Dims db, int transpose, int layer)
 {
     hipStream_t s = u->ctx->stream;
-    const int chunk = 256;
-    const int nb = (int)((da.n() + chunk - 1) / chunk);
+    const int nb = (int)((da.n() + kWChunk - 1) / kWChunk);
     const size_t need = (size_t)nb * kC * cb * 27;
     if (need > u->partial_elems) {
         sgpu_set_error("unet: weight-gradient partial buffer too small");
         return SGPU_ERR_STATE;
     }
     if (stride == 1 && cb == kC)
-        wgrad<1, kC><<<nb, 256, 0, s>>>(A, da, B, db, chunk, u->partial);
+        wgrad<1, kC><<<nb, 256, 0, s>>>(A, da, B, db, u->partial);
     else if (stride == 2 && cb == kC)
-        wgrad<2, kC><<<nb, 256, 0, s>>>(A, da, B, db, chunk, u->partial);
+        wgrad<2, kC><<<nb, 256, 0, s>>>(A, da, B, db, u->partial);
     else if (stride == 2 && cb == 1)
-        wgrad<2, 1><<<nb, 256, 0, s>>>(A, da, B, db, chunk, u->partial);
+        wgrad<2, 1><<<nb, 256, 0, s>>>(A, da, B, db, u->partial);
     else {
         sgpu_set_error("unet: unsupported weight-gradient shape");
         return SGPU_ERR_ARG;
@@ -942,7 +977,7 @@ extern "C" int sgpu_unet_create(sgpu_context *ctx, int channels, unsigned long l
         if ((st = dalloc(u, p, n2))) return fail(st);
     if ((st = dalloc(u, &u->f32buf, (size_t)u->d0.n())) || (st = dalloc(u, &u->lpart, 256)))
         return fail(st);
-    u->partial_elems = (size_t)((u->d1.n() + 255) / 256) * kC * kC * 27;
+    u->partial_elems = (size_t)((u->d1.n() + kWChunk - 1) / kWChunk) * kC * kC * 27;
     if ((st = dalloc(u, &u->partial, u->partial_elems))) return fail(st);
     // weights: layers 0..6 uniform(-a, a) with a = sqrt(3 / fan_in) * 0.5
     // (splitmix64 stream from seed), the last layer zero so that the initial
