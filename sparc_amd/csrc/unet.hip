@@ -775,6 +775,7 @@ struct sgpu_unet {
     int off[kLayers + 1] = {0};
     float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr, *wt = nullptr;
     uint16_t *wb = nullptr;   // layers 1..6 packed for the matrix cores (pack_bf16)
+    bool wb_current = false;  // wb holds the packed current weights (cleared on every write of w)
     uint16_t *wtb = nullptr;  // wt packed likewise
     // activations (BF16 channels-last) and their gradients
     uint16_t *a1 = nullptr, *b1 = nullptr, *a2 = nullptr, *b2 = nullptr, *c2 = nullptr,
@@ -812,8 +813,13 @@ int forward(sgpu_unet *u, const TI *r, TO *f, double out_scale)
 {
     hipStream_t s = u->ctx->stream;
     const float *W = u->w;
-    // layers 1..6 (C -> C, contiguous in W) packed for the matrix cores
-    pack_bf16<<<nblk(6L * kWcc), 256, 0, s>>>(layer_w(u, W, 1), 6, u->wb);
+    // layers 1..6 (C -> C, contiguous in W) packed for the matrix cores, once
+    // per change of the weights (stream order makes the pack visible to every
+    // later pass)
+    if (!u->wb_current) {
+        pack_bf16<<<nblk(6L * kWcc), 256, 0, s>>>(layer_w(u, W, 1), 6, u->wb);
+        u->wb_current = true;
+    }
     auto wb = [&](int l) -> const uint16_t * { return u->wb + (long)(l - 1) * kWcc; };
     conv_1c<TI><<<nblk(u->d1.n()), 256, 0, s>>>(r, u->d0, u->a1, u->d1, layer_w(u, W, 0), 1.f);
     launch_conv_cc<1>(u->a1, u->d1, u->b1, u->d1, layer_w(u, W, 1), wb(1), nullptr, s);
@@ -998,6 +1004,7 @@ extern "C" int sgpu_unet_create(sgpu_context *ctx, int channels, unsigned long l
     }
     SGPU_HIP(hipMemcpyAsync(u->w, hw.data(), hw.size() * sizeof(float), hipMemcpyHostToDevice,
                             ctx->stream));
+    u->wb_current = false;
     SGPU_HIP(hipMemsetAsync(u->m, 0, u->nparam * sizeof(float), ctx->stream));
     SGPU_HIP(hipMemsetAsync(u->v, 0, u->nparam * sizeof(float), ctx->stream));
     SGPU_HIP(hipStreamSynchronize(ctx->stream));
@@ -1029,6 +1036,7 @@ extern "C" int sgpu_unet_set_weights(sgpu_unet *u, const float *w)
 {
     if (!u || !w) return SGPU_ERR_ARG;
     SGPU_HIP(hipMemcpyAsync(u->w, w, u->nparam * sizeof(float), hipMemcpyHostToDevice, u->ctx->stream));
+    u->wb_current = false;
     SGPU_HIP(hipMemsetAsync(u->m, 0, u->nparam * sizeof(float), u->ctx->stream));
     SGPU_HIP(hipMemsetAsync(u->v, 0, u->nparam * sizeof(float), u->ctx->stream));
     SGPU_HIP(hipStreamSynchronize(u->ctx->stream));
@@ -1241,6 +1249,7 @@ extern "C" int sgpu_unet_train(sgpu_unet *u, int steps, int batch, double lr,
         const float c1 = 1.f - std::pow(b1, (float)u->adam_t), c2 = 1.f - std::pow(b2, (float)u->adam_t);
         adam_step<<<nblk(u->nparam), 256, 0, u->ctx->stream>>>(u->w, u->g, u->m, u->v, u->nparam,
                                                                (float)lr, b1, b2, 1e-8f, c1, c2);
+        u->wb_current = false;
         SGPU_HIP(hipGetLastError());
     }
     SGPU_HIP(hipStreamSynchronize(u->ctx->stream));
