@@ -284,7 +284,10 @@ int sgpu_aar_device(sgpu_context *ctx, sgpu_res_fn res, sgpu_precond_fn precond,
             pa[0] = r; pb[0] = r;
             SGPU_CHECK(sgpu_dot_many(ctx, N, 1, pa, pb, &rnorm));
             rnorm = std::sqrt(rnorm);
-            if (!std::isfinite(rnorm)) break;  // diverged: reported as not converged
+            // diverged (not finite, or 1e6 x the right-hand side's norm, which
+            // no converging solve reaches): reported as not converged at once
+            // instead of after max_iter iterations
+            if (!std::isfinite(rnorm) || (bnorm > 0 && rnorm > 1e6 * bnorm)) break;
         } else {
             SGPU_CHECK(residual());
         }
@@ -295,8 +298,7 @@ int sgpu_aar_device(sgpu_context *ctx, sgpu_res_fn res, sgpu_precond_fn precond,
     SGPU_CHECK(sgpu_dot_many(ctx, N, 1, pa, pb, &r2));
     if (relres) *relres = bnorm > 0 ? std::sqrt(r2) / bnorm : 0.0;
     if (iters) *iters = it;
-    return (!std::isfinite(rnorm) || (rnorm > atol && it >= max_iter)) ? SGPU_NOT_CONVERGED
-                                                                       : SGPU_OK;
+    return (!std::isfinite(rnorm) || rnorm > atol) ? SGPU_NOT_CONVERGED : SGPU_OK;
 }
 
 extern "C" int sgpu_AAR(sgpu_context *ctx, sgpu_res_fn res, sgpu_precond_fn precond, void *user,
