@@ -601,22 +601,20 @@ __global__ __launch_bounds__(256) void wgrad(const uint16_t *__restrict__ A, Dim
 __global__ void wgrad_reduce(const float *__restrict__ partial, int nblocks, int na, int nb,
                              int transpose, float *__restrict__ dW)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    // eight lanes per output (g = lane % 8, all in one wave): lane g sums the
+    // blocks b = g mod 8 in ascending order, then the eight sums are combined
+    // pairwise, ((s0 + s1) + (s2 + s3)) + ((s4 + s5) + (s6 + s7)), by an xor
+    // butterfly (the same sum on every lane: addition is commutative)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = na * nb * 27;
-    if (i >= n) return;
-    // eight interleaved running sums (blocks b = u mod 8), combined pairwise:
-    // eight loads in flight per thread instead of one
-    float su[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    int b = 0;
-    for (; b + 8 <= nblocks; b += 8) {
-        float v[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) v[u] = partial[(long)(b + u) * n + i];
-#pragma unroll
-        for (int u = 0; u < 8; u++) su[u] += v[u];
-    }
-    for (int u = 0; b < nblocks; b++, u++) su[u] += partial[(long)b * n + i];
-    const float s = ((su[0] + su[1]) + (su[2] + su[3])) + ((su[4] + su[5]) + (su[6] + su[7]));
+    const int i = min(t >> 3, n - 1), g = t & 7;
+    float s = 0.f;
+#pragma unroll 8
+    for (int b = g; b < nblocks; b += 8) s += partial[(long)b * n + i];
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    s += __shfl_xor(s, 4);
+    if ((t >> 3) >= n || g != 0) return;
     const int k = i % 27, bb = (i / 27) % nb, a = i / (27 * nb);
     const int o = transpose ? (bb * na + a) * 27 + k : i;
     dW[o] += s;
@@ -855,7 +853,7 @@ int wgrad_launch(sgpu_unet *u, int stride, const uint16_t *A, Dims da, const voi
         return SGPU_ERR_ARG;
     }
     const int n = kC * cb * 27;
-    wgrad_reduce<<<nblk(n), 256, 0, s>>>(u->partial, nb, kC, cb, transpose, u->g + u->off[layer]);
+    wgrad_reduce<<<nblk(8L * n), 256, 0, s>>>(u->partial, nb, kC, cb, transpose, u->g + u->off[layer]);
     SGPU_HIP(hipGetLastError());
     return SGPU_OK;
 }
