@@ -775,6 +775,9 @@ struct sgpu_unet {
     uint16_t *wb = nullptr;   // layers 1..6 packed for the matrix cores (pack_bf16)
     bool wb_current = false;  // wb holds the packed current weights (cleared on every write of w)
     uint16_t *wtb = nullptr;  // wt packed likewise
+    // wt / wtb hold the transforms of all eight layers (layer l at l * kWcc),
+    // formed once per change of the weights, not once per pair
+    bool wt_current = false;
     // activations (BF16 channels-last) and their gradients
     uint16_t *a1 = nullptr, *b1 = nullptr, *a2 = nullptr, *b2 = nullptr, *c2 = nullptr,
              *s1 = nullptr, *c1 = nullptr;
@@ -864,12 +867,24 @@ int backward(sgpu_unet *u, const float *r, const float *df)
 {
     hipStream_t s = u->ctx->stream;
     const float *W = u->w;
-    float *wt = u->wt;
+    // the adjoint weights of layers 1..7 (mode 0: stride-1 conv adjoint, 1:
+    // stride-2 conv <-> transposed conv), formed when the weights changed
+    static const int kMode[kLayers] = {0, 0, 1, 0, 0, 1, 0, 1};
+    if (!u->wt_current) {
+        for (int l = 1; l < kLayers; l++) {
+            wtransform<<<nblk(layer_size(l)), 256, 0, s>>>(layer_w(u, W, l), kLayerCout[l],
+                                                            kLayerCin[l], kMode[l], u->wt + (long)l * kWcc);
+            if (kLayerCin[l] == kC && kLayerCout[l] == kC)
+                pack_bf16<<<nblk(kWcc), 256, 0, s>>>(u->wt + (long)l * kWcc, 1, u->wtb + (long)l * kWcc);
+        }
+        u->wt_current = true;
+    }
+    const float *wt = nullptr;
+    const uint16_t *wtb = nullptr;
     auto transform = [&](int l, int mode) {
-        wtransform<<<nblk(layer_size(l)), 256, 0, s>>>(layer_w(u, W, l), kLayerCout[l], kLayerCin[l],
-                                                        mode, wt);
-        if (kLayerCin[l] == kC && kLayerCout[l] == kC)
-            pack_bf16<<<nblk(kWcc), 256, 0, s>>>(wt, 1, u->wtb);
+        (void)mode;  // kMode[l]
+        wt = u->wt + (long)l * kWcc;
+        wtb = u->wtb + (long)l * kWcc;
     };
     // f = (r + U) / |D0 + shift|  ->  dU = df / |D0 + shift|, applied by loss_grad
     float *dU = u->f32buf;  // overwrite df copy with dU
@@ -883,28 +898,28 @@ int backward(sgpu_unet *u, const float *r, const float *df)
     // C1 (conv C->C, L1): dW6 = sum_y dc1[y][co] s1[y+k-1][ci]
     SGPU_CHECK(wgrad_launch(u, 1, u->ga, u->d1, u->s1, kC, u->d1, 0, 6));
     transform(6, 0);
-    launch_conv_cc<1>(u->ga, u->d1, u->gb, u->d1, wt, u->wtb, nullptr, s);  // ds1
+    launch_conv_cc<1>(u->ga, u->d1, u->gb, u->d1, wt, wtb, nullptr, s);  // ds1
     // P2 (convT L2->L1): dW5[co][ci][k] = sum_y c2[y][ci] ds1[2y+k-1][co] -> transpose
     SGPU_CHECK(wgrad_launch(u, 2, u->c2, u->d2, u->gb, kC, u->d1, 1, 5));
     transform(5, 1);
-    launch_conv_cc<2>(u->gb, u->d1, u->gc, u->d2, wt, u->wtb, nullptr, s);  // dc2
+    launch_conv_cc<2>(u->gb, u->d1, u->gc, u->d2, wt, wtb, nullptr, s);  // dc2
     // M2 (conv L2): dW4 = sum dc2[y][co] b2[y+k-1][ci]
     SGPU_CHECK(wgrad_launch(u, 1, u->gc, u->d2, u->b2, kC, u->d2, 0, 4));
     transform(4, 0);
-    launch_conv_cc<1>(u->gc, u->d2, u->gd, u->d2, wt, u->wtb, nullptr, s);  // db2
+    launch_conv_cc<1>(u->gc, u->d2, u->gd, u->d2, wt, wtb, nullptr, s);  // db2
     // B2 (conv L2): dW3 = sum db2[y][co] a2[y+k-1][ci]
     SGPU_CHECK(wgrad_launch(u, 1, u->gd, u->d2, u->a2, kC, u->d2, 0, 3));
     transform(3, 0);
-    launch_conv_cc<1>(u->gd, u->d2, u->gc, u->d2, wt, u->wtb, nullptr, s);  // da2
+    launch_conv_cc<1>(u->gd, u->d2, u->gc, u->d2, wt, wtb, nullptr, s);  // da2
     // R2 (conv/2 L1->L2): dW2 = sum da2[y][co] b1[2y+k-1][ci]
     SGPU_CHECK(wgrad_launch(u, 2, u->gc, u->d2, u->b1, kC, u->d1, 0, 2));
     // db1 = ds1 (skip) + R2 adjoint(da2): transposed conv with W'[ci][co][k] = W2[co][ci][k]
     transform(2, 1);
-    launch_convT_cc(u->gc, u->d2, u->ga, u->d1, wt, u->wtb, u->gb, s);  // db1
+    launch_convT_cc(u->gc, u->d2, u->ga, u->d1, wt, wtb, u->gb, s);  // db1
     // B1 (conv L1): dW1 = sum db1[y][co] a1[y+k-1][ci]
     SGPU_CHECK(wgrad_launch(u, 1, u->ga, u->d1, u->a1, kC, u->d1, 0, 1));
     transform(1, 0);
-    launch_conv_cc<1>(u->ga, u->d1, u->gb, u->d1, wt, u->wtb, nullptr, s);  // da1
+    launch_conv_cc<1>(u->ga, u->d1, u->gb, u->d1, wt, wtb, nullptr, s);  // da1
     // R1 (conv/2 1->C, L0->L1): dW0[co][0][k] = sum_y da1[y][co] r[2y+k-1]
     SGPU_CHECK(wgrad_launch(u, 2, u->gb, u->d1, r, 1, u->d0, 0, 0));
     SGPU_HIP(hipGetLastError());
@@ -969,8 +984,8 @@ extern "C" int sgpu_unet_create(sgpu_context *ctx, int channels, unsigned long l
     int st;
     if ((st = dalloc(u, &u->w, u->nparam)) || (st = dalloc(u, &u->g, u->nparam)) ||
         (st = dalloc(u, &u->m, u->nparam)) || (st = dalloc(u, &u->v, u->nparam)) ||
-        (st = dalloc(u, &u->wt, (size_t)kC * kC * 27)) || (st = dalloc(u, &u->wb, 6 * (size_t)kWcc)) ||
-        (st = dalloc(u, &u->wtb, (size_t)kWcc)))
+        (st = dalloc(u, &u->wt, (size_t)kLayers * kWcc)) || (st = dalloc(u, &u->wb, 6 * (size_t)kWcc)) ||
+        (st = dalloc(u, &u->wtb, (size_t)kLayers * kWcc)))
         return fail(st);
     const size_t n1 = (size_t)u->d1.n() * kC, n2 = (size_t)u->d2.n() * kC;
     uint16_t **l1[] = {&u->a1, &u->b1, &u->s1, &u->c1, &u->ga, &u->gb};
@@ -1003,6 +1018,7 @@ extern "C" int sgpu_unet_create(sgpu_context *ctx, int channels, unsigned long l
     SGPU_HIP(hipMemcpyAsync(u->w, hw.data(), hw.size() * sizeof(float), hipMemcpyHostToDevice,
                             ctx->stream));
     u->wb_current = false;
+    u->wt_current = false;
     SGPU_HIP(hipMemsetAsync(u->m, 0, u->nparam * sizeof(float), ctx->stream));
     SGPU_HIP(hipMemsetAsync(u->v, 0, u->nparam * sizeof(float), ctx->stream));
     SGPU_HIP(hipStreamSynchronize(ctx->stream));
@@ -1035,6 +1051,7 @@ extern "C" int sgpu_unet_set_weights(sgpu_unet *u, const float *w)
     if (!u || !w) return SGPU_ERR_ARG;
     SGPU_HIP(hipMemcpyAsync(u->w, w, u->nparam * sizeof(float), hipMemcpyHostToDevice, u->ctx->stream));
     u->wb_current = false;
+    u->wt_current = false;
     SGPU_HIP(hipMemsetAsync(u->m, 0, u->nparam * sizeof(float), u->ctx->stream));
     SGPU_HIP(hipMemsetAsync(u->v, 0, u->nparam * sizeof(float), u->ctx->stream));
     SGPU_HIP(hipStreamSynchronize(u->ctx->stream));
@@ -1248,6 +1265,7 @@ extern "C" int sgpu_unet_train(sgpu_unet *u, int steps, int batch, double lr,
         adam_step<<<nblk(u->nparam), 256, 0, u->ctx->stream>>>(u->w, u->g, u->m, u->v, u->nparam,
                                                                (float)lr, b1, b2, 1e-8f, c1, c2);
         u->wb_current = false;
+        u->wt_current = false;
         SGPU_HIP(hipGetLastError());
     }
     SGPU_HIP(hipStreamSynchronize(u->ctx->stream));
