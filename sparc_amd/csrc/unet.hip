@@ -558,9 +558,11 @@ __global__ __launch_bounds__(256) void wgrad(const uint16_t *__restrict__ A, Dim
         float bv[kWB][PPT];
 #pragma unroll
         for (int u = 0; u < kWB; u++) {
-            const long y = y0 + min(yb + u, ny - 1);  // tail: a valid voxel, not accumulated
-            const int cx = (int)(y % da.nx), cy = (int)((y / da.nx) % da.ny),
-                      cz = (int)(y / ((long)da.nx * da.ny));
+            // tail: a valid voxel, not accumulated; 32-bit index arithmetic
+            // (a coarse level holds far fewer than 2^31 voxels)
+            const int y = (int)y0 + min(yb + u, ny - 1);
+            const int yq = y / da.nx;
+            const int cx = y - yq * da.nx, cy = yq % da.ny, cz = yq / da.ny;
 #pragma unroll
             for (int p = 0; p < PPT; p++) {
                 const int k = pk_k[p], b = pk_b[p];
@@ -962,6 +964,10 @@ extern "C" int sgpu_unet_create(sgpu_context *ctx, int channels, unsigned long l
     if (ctx->nx % 4 || ctx->ny % 4 || ctx->nz % 4 || ctx->bcx || ctx->bcy || ctx->bcz) {
         sgpu_set_error("sgpu_unet_create: needs a periodic grid with every dimension a multiple "
                        "of 4 (got %d x %d x %d)", ctx->nx, ctx->ny, ctx->nz);
+        return SGPU_ERR_ARG;
+    }
+    if (ctx->nd >= (1L << 31)) {  // the kernels index a level's voxels with int
+        sgpu_set_error("sgpu_unet_create: grid of %ld points exceeds 2^31 - 1", ctx->nd);
         return SGPU_ERR_ARG;
     }
     SGPU_HIP(hipSetDevice(ctx->device));
