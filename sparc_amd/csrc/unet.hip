@@ -101,7 +101,8 @@ __global__ __launch_bounds__(256) void conv_cc(const uint16_t *__restrict__ in, 
     stage_weights(W, kC, kC, wl);
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= dout.n()) return;
-    const int ox = (int)(v % dout.nx), oy = (int)((v / dout.nx) % dout.ny), oz = (int)(v / ((long)dout.nx * dout.ny));
+    const int vq = (int)v / dout.nx;  // 32-bit: a level holds < 2^31 voxels
+    const int ox = (int)v - vq * dout.nx, oy = vq % dout.ny, oz = vq / dout.ny;
     float acc[kC];
 #pragma unroll
     for (int c = 0; c < kC; c++) acc[c] = 0.f;
@@ -138,7 +139,8 @@ __global__ __launch_bounds__(256) void convT_cc(const uint16_t *__restrict__ in,
     stage_weights(W, kC, kC, wl);
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= dout.n()) return;
-    const int fx = (int)(v % dout.nx), fy = (int)((v / dout.nx) % dout.ny), fz = (int)(v / ((long)dout.nx * dout.ny));
+    const int vq = (int)v / dout.nx;  // 32-bit: a level holds < 2^31 voxels
+    const int fx = (int)v - vq * dout.nx, fy = vq % dout.ny, fz = vq / dout.ny;
     int kxs[2], cxs[2], kys[2], cys[2], kzs[2], czs[2];
     const int nkx = (fx & 1) ? 2 : 1, nky = (fy & 1) ? 2 : 1, nkz = (fz & 1) ? 2 : 1;
     if (fx & 1) { kxs[0] = 0; cxs[0] = wrap((fx + 1) / 2, di.nx); kxs[1] = 2; cxs[1] = (fx - 1) / 2; }
@@ -429,7 +431,8 @@ __global__ __launch_bounds__(256) void conv_1c(const TI *__restrict__ in, Dims d
     stage_weights(W, 1, kC, wl);
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= dout.n()) return;
-    const int ox = (int)(v % dout.nx), oy = (int)((v / dout.nx) % dout.ny), oz = (int)(v / ((long)dout.nx * dout.ny));
+    const int vq = (int)v / dout.nx;  // 32-bit: a level holds < 2^31 voxels
+    const int ox = (int)v - vq * dout.nx, oy = vq % dout.ny, oz = vq / dout.ny;
     float acc[kC];
 #pragma unroll
     for (int c = 0; c < kC; c++) acc[c] = 0.f;
@@ -469,7 +472,8 @@ __global__ __launch_bounds__(256) void convT_c1_out_blk(const uint16_t *__restri
     stage_weights(W, kC, 1, wl);
     const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= di.n()) return;
-    const int cx = (int)(v % di.nx), cy = (int)((v / di.nx) % di.ny), cz = (int)(v / ((long)di.nx * di.ny));
+    const int vq = (int)v / di.nx;  // 32-bit: a level holds < 2^31 voxels
+    const int cx = (int)v - vq * di.nx, cy = vq % di.ny, cz = vq / di.ny;
     // coarse voxel (cx + dx, cy + dy, cz + dz), d in {0,1}: 16 BF16 channels as 8 words
     uint32_t cw[8][8];
 #pragma unroll
