@@ -54,8 +54,10 @@ typedef struct sgpu_block sgpu_block;
  * second-derivative weights of src/initialization.c:2113-2147. */
 int sgpu_create(sgpu_context **ctx, int device, int nx, int ny, int nz,
                 double hx, double hy, double hz, int bcx, int bcy, int bcz, int order);
-/* Destroying a context releases the storage of its remaining blocks; their
- * handles stay valid only for sgpu_block_destroy. */
+/* Destroying a context releases the storage of its remaining blocks and
+ * U-Nets; their handles stay valid only for sgpu_block_destroy /
+ * sgpu_unet_destroy (any other call on them returns SGPU_ERR_STATE).  Any
+ * teardown order is safe. */
 int sgpu_destroy(sgpu_context *ctx);
 const char *sgpu_last_error(void);
 /* Scaled D2 stencil coefficients D2_stencil_coeffs_{x,y,z}[0..order/2]. */

@@ -121,6 +121,8 @@ extern "C" int sgpu_destroy(sgpu_context *ctx)
         b->ctx = nullptr;  // orphaned: sgpu_block_destroy only frees the handle
     }
     ctx->blocks.clear();
+    for (sgpu_unet *u : ctx->unets) sgpu_unet_orphan(u);
+    ctx->unets.clear();
     sgpu_free_nonlocal(ctx->nl);
     if (ctx->veff64) (void)hipFree(ctx->veff64);
     if (ctx->veff32) (void)hipFree(ctx->veff32);

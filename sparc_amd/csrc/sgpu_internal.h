@@ -163,6 +163,8 @@ struct sgpu_context {
     std::vector<hipEvent_t> ev_pool;
     // live blocks; sgpu_destroy releases their storage and orphans them
     std::vector<sgpu_block *> blocks;
+    // live U-Nets; sgpu_destroy releases their device memory and orphans them
+    std::vector<sgpu_unet *> unets;
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X (linear
@@ -210,6 +212,10 @@ int sgpu_launch_vnl(sgpu_context *ctx, int dtype, int ncol, const void *x, long 
 int sgpu_launch_vnl_chain(sgpu_context *ctx, int dtype, int ncol, const void *x, long ldx,
                           void *out, long ldo, double scale, bool have_part, bool want_next,
                           bool *projected);
+// ---- unet.hip ----
+// Release a network's device memory and detach it from its context (called
+// by sgpu_destroy; the handle then only accepts sgpu_unet_destroy).
+void sgpu_unet_orphan(sgpu_unet *u);
 // ---- aar.hip ----
 void anderson_coeff(int m, const double *FtF, const double *Ftf, double *G);
 int sgpu_anderson_coeff_dev(sgpu_context *ctx, const double *F, long ldf, int m, const double *f,

@@ -1032,17 +1032,58 @@ extern "C" int sgpu_unet_create(sgpu_context *ctx, int channels, unsigned long l
     SGPU_HIP(hipMemsetAsync(u->m, 0, u->nparam * sizeof(float), ctx->stream));
     SGPU_HIP(hipMemsetAsync(u->v, 0, u->nparam * sizeof(float), ctx->stream));
     SGPU_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->unets.push_back(u);
     *out = u;
     return SGPU_OK;
+}
+
+namespace {
+
+void unet_free_device(sgpu_unet *u)
+{
+    for (void *p : u->owned) (void)hipFree(p);
+    u->owned.clear();
+    if (u->pr) (void)hipFree(u->pr);
+    if (u->pe) (void)hipFree(u->pe);
+    u->pr = u->pe = nullptr;
+    u->npairs = u->cap = 0;
+}
+
+// Entry points other than sgpu_unet_destroy need the network's context.
+int unet_live(const sgpu_unet *u, const char *what)
+{
+    if (!u) {
+        sgpu_set_error("%s: null network", what);
+        return SGPU_ERR_ARG;
+    }
+    if (!u->ctx) {
+        sgpu_set_error("%s: the network's context was destroyed", what);
+        return SGPU_ERR_STATE;
+    }
+    return SGPU_OK;
+}
+
+}  // namespace
+
+// Called by sgpu_destroy (the context's stream is already synchronised): the
+// context's stream and device memory go away with it, so the network keeps
+// only its host handle.
+void sgpu_unet_orphan(sgpu_unet *u)
+{
+    if (!u) return;
+    unet_free_device(u);
+    u->ctx = nullptr;
 }
 
 extern "C" void sgpu_unet_destroy(sgpu_unet *u)
 {
     if (!u) return;
-    if (u->ctx) (void)hipStreamSynchronize(u->ctx->stream);
-    for (void *p : u->owned) (void)hipFree(p);
-    if (u->pr) (void)hipFree(u->pr);
-    if (u->pe) (void)hipFree(u->pe);
+    if (u->ctx) {
+        (void)hipStreamSynchronize(u->ctx->stream);
+        auto &v = u->ctx->unets;
+        v.erase(std::remove(v.begin(), v.end(), u), v.end());
+    }
+    unet_free_device(u);
     delete u;
 }
 
@@ -1050,7 +1091,8 @@ extern "C" int sgpu_unet_nparams(const sgpu_unet *u) { return u ? u->nparam : 0;
 
 extern "C" int sgpu_unet_get_weights(const sgpu_unet *u, float *w)
 {
-    if (!u || !w) return SGPU_ERR_ARG;
+    SGPU_CHECK(unet_live(u, "sgpu_unet_get_weights"));
+    if (!w) return SGPU_ERR_ARG;
     SGPU_HIP(hipMemcpyAsync(w, u->w, u->nparam * sizeof(float), hipMemcpyDeviceToHost, u->ctx->stream));
     SGPU_HIP(hipStreamSynchronize(u->ctx->stream));
     return SGPU_OK;
@@ -1058,7 +1100,8 @@ extern "C" int sgpu_unet_get_weights(const sgpu_unet *u, float *w)
 
 extern "C" int sgpu_unet_set_weights(sgpu_unet *u, const float *w)
 {
-    if (!u || !w) return SGPU_ERR_ARG;
+    SGPU_CHECK(unet_live(u, "sgpu_unet_set_weights"));
+    if (!w) return SGPU_ERR_ARG;
     SGPU_HIP(hipMemcpyAsync(u->w, w, u->nparam * sizeof(float), hipMemcpyHostToDevice, u->ctx->stream));
     u->wb_current = false;
     u->wt_current = false;
@@ -1076,14 +1119,15 @@ extern "C" int sgpu_unet_set_weights(sgpu_unet *u, const float *w)
 extern "C" void sgpu_unet_precond(void *user, int N, double c, const double *r_dev, double *f_dev)
 {
     sgpu_unet *u = static_cast<sgpu_unet *>(user);
-    if (!u || N != u->d0.nx * u->d0.ny * u->d0.nz) return;
+    if (!u || !u->ctx || N != u->d0.nx * u->d0.ny * u->d0.nz) return;
     const double diag = std::fabs(u->d0sum + c);
     (void)forward<double, double>(u, r_dev, f_dev, diag > 0.0 ? 1.0 / diag : 1.0 / u->d0abs);
 }
 
 extern "C" int sgpu_unet_apply(sgpu_unet *u, const double *r_host, double *f_host)
 {
-    if (!u || !r_host || !f_host) return SGPU_ERR_ARG;
+    SGPU_CHECK(unet_live(u, "sgpu_unet_apply"));
+    if (!r_host || !f_host) return SGPU_ERR_ARG;
     const long n = u->d0.n();
     double *buf = nullptr;
     SGPU_HIP(hipMalloc((void **)&buf, 2 * n * sizeof(double)));
@@ -1105,7 +1149,8 @@ extern "C" int sgpu_unet_apply(sgpu_unet *u, const double *r_host, double *f_hos
 extern "C" int sgpu_unet_loss_grad(sgpu_unet *u, const double *r_host, const double *e_host,
                                    int npair, double *loss, float *grad)
 {
-    if (!u || !r_host || !e_host || npair < 1) return SGPU_ERR_ARG;
+    SGPU_CHECK(unet_live(u, "sgpu_unet_loss_grad"));
+    if (!r_host || !e_host || npair < 1) return SGPU_ERR_ARG;
     const long n = u->d0.n();
     float *buf = nullptr;
     SGPU_HIP(hipMalloc((void **)&buf, 2 * (size_t)npair * n * sizeof(float)));
@@ -1141,7 +1186,8 @@ extern "C" int sgpu_unet_loss_grad(sgpu_unet *u, const double *r_host, const dou
 extern "C" int sgpu_unet_harvest(sgpu_unet *u, const double *b_host, const double *x0_host,
                                  double tol, int max_iter, int stride, int max_pairs, int *added)
 {
-    if (!u || !b_host || !x0_host || stride < 1 || max_pairs < 1) return SGPU_ERR_ARG;
+    SGPU_CHECK(unet_live(u, "sgpu_unet_harvest"));
+    if (!b_host || !x0_host || stride < 1 || max_pairs < 1) return SGPU_ERR_ARG;
     sgpu_context *ctx = u->ctx;
     const long n = u->d0.n(), ld = ctx->ld;
     hipStream_t s = ctx->stream;
@@ -1246,7 +1292,8 @@ extern "C" int sgpu_unet_set_shift(sgpu_unet *u, double c)
 extern "C" int sgpu_unet_train(sgpu_unet *u, int steps, int batch, double lr,
                                unsigned long long seed, double *loss_hist)
 {
-    if (!u || steps < 0 || batch < 1 || u->npairs < 1) {
+    SGPU_CHECK(unet_live(u, "sgpu_unet_train"));
+    if (steps < 0 || batch < 1 || u->npairs < 1) {
         sgpu_set_error("sgpu_unet_train: bad arguments or no training pairs");
         return SGPU_ERR_ARG;
     }

@@ -241,7 +241,7 @@ class Context:
                                          grid.hy, grid.hz, grid.bcx, grid.bcy, grid.bcz, grid.order))
         self.h = h
         self._keep = []
-        self._nets = weakref.WeakSet()  # U-Nets hold the context's stream: closed first
+        self._nets = weakref.WeakSet()  # closed with the context (C: orphaned if left open)
         self._n_alpha = 0
 
     # -- plumbing

@@ -216,6 +216,13 @@ def load_case(base, psp_dir=None) -> Case:
         need(_i(keys, k, 0) == 0, f"{k}: not supported")
     unknown = [k for k in keys if k not in _IGNORED and k not in _KNOWN]
     need(not unknown, f"keys not supported: {', '.join(sorted(unknown))}")
+    # keys the reference refuses in combination (src/readfiles.c:1101-1120)
+    for group in (("FD_GRID", "MESH_SPACING", "ECUT"),
+                  ("TOL_SCF", "ACCURACY", "SCF_ENERGY_ACC", "SCF_FORCE_ACC"),
+                  ("SMEARING", "ELEC_TEMP")):
+        given = [k for k in group if k in keys]
+        if len(given) > 1:
+            raise ValueError(f"{base.name}: {', '.join(given)} cannot be simultaneously specified")
 
     # cell: LATVEC must be a scaled identity (orthogonal axes along x, y, z)
     lat = np.asarray([float(v) for v in keys.get("LATVEC", ["1", "0", "0", "0", "1", "0", "0", "0", "1"])])

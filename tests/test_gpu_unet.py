@@ -160,3 +160,37 @@ def test_unet_lds_staged_layers_match_direct_taps(monkeypatch):
             out[lds] = net.apply(r)
     assert np.isfinite(out["1"]).all()
     assert np.array_equal(out["1"], out["0"])
+
+
+def test_context_destroyed_before_unet_and_block(grid16):
+    """sgpu_destroy before sgpu_unet_destroy / sgpu_block_destroy (the order a
+    C caller with static handles may use): the context releases the network's
+    and the block's device memory and orphans both handles; every other call
+    on the orphaned network reports SGPU_ERR_STATE, and destroying the handles
+    afterwards is safe.  Calls go straight through the C ABI."""
+    import ctypes as C
+    from sparc_amd import gpu
+    lib = gpu.load()
+    g = grid16
+    ctx = C.c_void_p()
+    assert lib.sgpu_create(C.byref(ctx), 0, g.nx, g.ny, g.nz, g.hx, g.hy, g.hz, 0, 0, 0, 12) == 0
+    net, blk = C.c_void_p(), C.c_void_p()
+    assert lib.sgpu_unet_create(ctx, 16, 3, C.byref(net)) == 0
+    assert lib.sgpu_block_create(ctx, 2, gpu.F64, C.byref(blk)) == 0
+    n = lib.sgpu_unet_nparams(net)
+    w = np.zeros(n, np.float32)
+    wp = w.ctypes.data_as(C.POINTER(C.c_float))
+    assert lib.sgpu_unet_get_weights(net, wp) == 0
+    assert lib.sgpu_destroy(ctx) == 0
+    assert lib.sgpu_unet_get_weights(net, wp) == -4          # SGPU_ERR_STATE
+    assert b"destroyed" in lib.sgpu_last_error()
+    r = np.ones(g.nd)
+    f = np.zeros(g.nd)
+    assert lib.sgpu_unet_apply(net, r.ctypes.data_as(C.POINTER(C.c_double)),
+                               f.ctypes.data_as(C.POINTER(C.c_double))) == -4
+    lib.sgpu_unet_destroy(net)
+    assert lib.sgpu_block_destroy(blk) == 0
+    # a fresh context and network work after the orphaned handles are gone
+    with gpu.Context(g) as c2:
+        f2 = c2.unet(seed=5).apply(r)
+    assert np.allclose(f2, r / _d0abs(g), rtol=0, atol=1e-14)

@@ -100,6 +100,22 @@ def test_unsupported_physics_is_refused(tmp_path, extra, what):
         load_case(_write_case(tmp_path, extra))
 
 
+@pytest.mark.parametrize("extra,keys", [
+    ("MESH_SPACING: 0.6", "FD_GRID, MESH_SPACING"),
+    ("ECUT: 20", "FD_GRID, ECUT"),
+    ("TOL_SCF: 1e-6\nACCURACY: 4", "TOL_SCF, ACCURACY"),
+    ("SCF_ENERGY_ACC: 1e-5\nSCF_FORCE_ACC: 1e-3", "SCF_ENERGY_ACC, SCF_FORCE_ACC"),
+    ("SMEARING: 0.01\nELEC_TEMP: 300", "SMEARING, ELEC_TEMP"),
+])
+def test_conflicting_keys_are_refused(tmp_path, extra, keys):
+    """Keys the reference refuses together (src/readfiles.c:1101-1120): more
+    than one way of giving the mesh, the SCF accuracy or the smearing."""
+    from sparc_amd.sparc_io import load_case
+    with pytest.raises(ValueError, match="cannot be simultaneously specified") as e:
+        load_case(_write_case(tmp_path, extra))
+    assert keys in str(e.value)
+
+
 def test_non_orthogonal_cell_is_refused(tmp_path):
     from sparc_amd.sparc_io import UnsupportedInput, load_case
     with pytest.raises(UnsupportedInput, match="LATVEC"):
