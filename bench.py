@@ -36,6 +36,40 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "SCF-iteration time & filter HBM GB/s, mixed vs FP64, at ΔE≤1e-5 Ha/atom"
 HBM_PEAK_GBS = 8000.0
+# MI355X dense matrix-core peaks (TFLOP/s)
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
+
+
+def host_threads():
+    """(threads used by the CPU legs, host CPUs visible, CPU model).  The CPU
+    share of a one-GPU box is 16 threads, so at most 16 are used."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    model = ""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return min(16, avail), avail, model
+
+
+def mfma_entry(flops, ms, peak_key, issued_flops=None):
+    """Achieved TFLOP/s of a dense kernel against the matrix-core peak of its
+    arithmetic; issued_flops counts padded / split-precision MFMA work."""
+    if not ms or ms <= 0:
+        return None
+    tf = flops / (ms * 1e-3) / 1e12
+    out = {"ms": round(ms, 4), "tflops": round(tf, 3), "peak_tflops": PEAK_TFLOPS[peak_key],
+           "peak": peak_key, "frac": round(tf / PEAK_TFLOPS[peak_key], 4)}
+    if issued_flops is not None:
+        ti = issued_flops / (ms * 1e-3) / 1e12
+        out.update(issued_tflops=round(ti, 3), issued_frac=round(ti / PEAK_TFLOPS[peak_key], 4))
+    return out
 
 
 def algorithmic_bytes(nd, ns, m, s, chi_elems):
@@ -88,7 +122,7 @@ def time_filter(ctx, gpu, wl, dtype, steps, warmup, bounds):
     t1 = time.perf_counter()
     ctx.timer_enable(False)
     out = {"wall_ms": (t1 - t0) * 1e3 / max(steps, 1)}
-    for name in ("filter", "stencil", "nonlocal"):
+    for name in ("filter", "stencil", "nonlocal", "nl_project", "nl_apply"):
         tot, cnt = ctx.timer(name)
         out[name] = (tot, cnt)
     finite = bool(np.isfinite(X.download(1)).all())
@@ -98,12 +132,27 @@ def time_filter(ctx, gpu, wl, dtype, steps, warmup, bounds):
     return out
 
 
+def nonlocal_dense(ctx, wl, rec, peak_key):
+    """Matrix-core figures of the nonlocal projection and apply of one filter
+    step: useful FLOPs 2 * sum(Chi) * ncol per kernel, issued FLOPs 2 *
+    stored * ncol (the layout's zero padding included)."""
+    mode, stored = ctx.nonlocal_layout()
+    chi, ns = int(wl.nloc.chi.size), wl.ncol
+    out = {"layout": "bricks" if mode == 0 else "spheres", "stored_values": stored,
+           "chi_values": chi}
+    for key in ("nl_project", "nl_apply"):
+        tot, cnt = rec[key]
+        if cnt:
+            out[key] = mfma_entry(2.0 * chi * ns, tot / cnt, peak_key, 2.0 * stored * ns)
+    return out
+
+
 def cpu_baseline(wl, ncol_sample=48, degree=None):
     """FP64 oracle filter on the host cores over a bounded sample of the same
     workload (same grid, Veff, projectors; ncol_sample vectors)."""
     from oracle import oracle as O
     from sparc_amd.workloads import uniform_block
-    threads = min(16, os.cpu_count() or 1)
+    threads, avail, model = host_threads()
     O.set_threads(threads)
     o = O.Oracle(wl.grid, wl.veff, wl.nloc)
     m = degree or wl.degree
@@ -119,8 +168,10 @@ def cpu_baseline(wl, ncol_sample=48, degree=None):
     chi = int(wl.nloc.chi.size)
     gbs = algorithmic_bytes(wl.grid.nd, ncol_sample, m, 8, chi) / dt / 1e9
     return {"value": round(gbs, 3), "unit": "GB/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle filter, same grid/Veff/projectors, {ncol_sample} of {wl.ncol} "
-                      f"vectors, degree {m}, {dt:.2f} s"}
+            "host_cpus": avail, "cpu_model": model,
+            "sample": f"FP64 oracle filter (OpenMP over columns, {threads} of {avail} host CPUs: "
+                      f"the CPU share of one GPU), same grid/Veff/projectors, {ncol_sample} of "
+                      f"{wl.ncol} vectors, degree {m}, {dt:.2f} s"}
 
 
 def scf_bench(name, unet_weights=None, kerker_weights=None):
@@ -194,8 +245,28 @@ def scf_bench(name, unet_weights=None, kerker_weights=None):
                 tot, cnt = ctx.timer(k)
                 if cnt:
                     phases[k] = round(tot / r["scf_iters"], 3)
+            dense = {}
+            ns_, nd_ = p["nstates"], g.nd
+            pk = "fp64" if dt == gpu.F64 else "fp32"
+            gk = os.environ.get("SPARC_AMD_GRAM", "")
+            bf16_gram = dt == gpu.F32 and (gk == "bf16x3" or (gk != "rocblas" and ns_ < 256))
+            for key, nprod in (("project_gemm", 2), ("rotate_gemm", 1)):
+                tot, cnt = ctx.timer(key)
+                if cnt:
+                    flops = nprod * 2.0 * nd_ * ns_ * ns_
+                    if key == "project_gemm" and bf16_gram:
+                        e = mfma_entry(flops, tot / cnt, "bf16", 6 * flops)
+                        e["kernel"] = "gram_bf16x3"
+                    else:
+                        e = mfma_entry(flops, tot / cnt, pk)
+                        e["kernel"] = "rocblas"
+                    dense[key] = e
+            tot, cnt = ctx.timer("eigensolve")
+            if cnt:
+                dense["eigensolve"] = {"ms": round(tot / cnt, 3), "n": ns_}
             runs[tag] = r
             out[tag] = {"ms_per_scf_iter": round(r["mean_iter_ms"], 3),
+                        "dense_per_call": dense,
                         "total_ms": round(r["total_ms"], 1), "scf_iters": r["scf_iters"],
                         "converged": bool(r["converged"]), "Etot": r["Etot"],
                         "Etot_per_atom": r["Etot"] / st.n_atom,
@@ -345,7 +416,7 @@ def cpu_baseline_scf(name="si8"):
     from oracle.scf import OracleSCF
     from sparc_amd.systems import scf_system
     from sparc_amd.workloads import uniform_block
-    threads = min(16, os.cpu_count() or 1)
+    threads, avail, _ = host_threads()
     O.set_threads(threads)
     st, p = scf_system(name)
     nd = st.grid.nd
@@ -355,10 +426,39 @@ def cpu_baseline_scf(name="si8"):
     r = OracleSCF(st.grid, st.nloc, st.b, st.rho_at, p, psi0, x0, rho_core=st.rho_core).run()
     dt = time.perf_counter() - t0
     return {"value": round(dt * 1e3 / r["scf_iters"], 2), "unit": "ms/SCF-iteration",
-            "cores": threads, "kind": "port",
+            "cores": threads, "kind": "port", "host_cpus": avail,
             "sample": f"FP64 oracle SCF, {name} (8-atom Si, psp8, 26^3, 24 states), whole run: "
                       f"{r['scf_iters']} iterations in {dt:.2f} s (mean includes the first "
                       f"iteration)"}
+
+
+def cpu_baseline_scf_iteration(name="si64", n_iter=2):
+    """FP64 oracle SCF of the 64-atom Si cell on the host cores, run for
+    n_iter iterations; the time of the last one is the per-iteration figure
+    (iterations after the first hold one CheFSI pass, like the device's
+    iterations 2..N)."""
+    from oracle import oracle as O
+    from oracle.scf import OracleSCF
+    from sparc_amd.systems import scf_system
+    from sparc_amd.workloads import uniform_block
+    threads, avail, _ = host_threads()
+    O.set_threads(threads)
+    st, p = scf_system(name)
+    nd = st.grid.nd
+    psi0 = uniform_block(nd, p["nstates"], p["psi_seed"], -0.5, 0.5)
+    x0 = uniform_block(nd, 1, p["lanczos_seed"], 0.0, 1.0)
+    s = OracleSCF(st.grid, st.nloc, st.b, st.rho_at, p, psi0, x0, rho_core=st.rho_core)
+    times = []
+    for k in range(n_iter):
+        t0 = time.perf_counter()
+        s.step(k)
+        times.append(time.perf_counter() - t0)
+    return {"value": round(times[-1] * 1e3, 1), "unit": "ms/SCF-iteration", "cores": threads,
+            "kind": "port", "host_cpus": avail,
+            "sample": f"FP64 oracle SCF, {name} ({st.n_atom}-atom Si, psp8, {st.grid.nx}^3, "
+                      f"{p['nstates']} states, degree {p['cheb_degree']}): iteration {n_iter} "
+                      f"(one CheFSI pass, Poisson and Kerker solves); iteration 1 (Lanczos + "
+                      f"{p['rho_trigger']} passes) took {times[0]:.2f} s"}
 
 
 def main(argv=None):
@@ -460,11 +560,16 @@ def main(argv=None):
                      "kernel": kernel,
                      "per_launch_algorithmic_bytes": stencil_bytes(nd, ns, m, 4) / m},
         "finite": r32["finite"],
+        # the whole FP32 filter (stencil + nonlocal launches) against HBM peak
+        "filter_roofline": {"bound": "hbm", "achieved": round(gbs32, 2), "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": round(gbs32 / HBM_PEAK_GBS, 4)},
+        "dense": {"nonlocal_f32": nonlocal_dense(ctx, wl, r32, "fp32")},
     }
     if not args.no_fp64:
         r64 = time_filter(ctx, gpu, wl, gpu.F64, args.steps, args.warmup, bounds)
         ms64 = r64["filter"][0] / max(r64["filter"][1], 1)
         gbs64 = algorithmic_bytes(nd, ns, m, 8, chi) / (ms64 * 1e-3) / 1e9
+        res["dense"]["nonlocal_f64"] = nonlocal_dense(ctx, wl, r64, "fp64")
         res["filter_ms"]["f64"] = round(ms64, 3)
         res["filter_gbs"]["f64"] = round(gbs64, 2)
         res["speedup_f32_vs_f64"] = round(ms64 / ms32, 3)
@@ -484,6 +589,7 @@ def main(argv=None):
         print("[bench] cpu baseline ...", flush=True)
         res["cpu_baseline"] = cpu_baseline(wl)
         res["cpu_baseline_scf"] = cpu_baseline_scf()
+        res["cpu_baseline_scf_si64"] = cpu_baseline_scf_iteration()
     print(json.dumps(res))
     return 0
 

@@ -88,6 +88,12 @@ int sgpu_set_nonlocal(sgpu_context *ctx, int n_influence, const int *ndc, const 
                       const double *chi, const int *alpha_off, int n_alpha,
                       const double *gamma, double dV);
 
+/* Diagnostic, no counterpart in the reference: the projector layout that
+ * sgpu_set_nonlocal chose (0 bricks of 16 x 8 x 4 points, 1 per-atom spheres)
+ * and the projector values it stores per copy, zero padding included; one
+ * projection or one apply issues 2 x stored multiply-adds per column. */
+int sgpu_nonlocal_layout(const sgpu_context *ctx, int *mode, long *stored);
+
 /* ---- device blocks -------------------------------------------------- */
 
 /* A block of ncol grid vectors in HBM, column-major, dtype SGPU_F64/F32. */

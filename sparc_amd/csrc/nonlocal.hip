@@ -810,7 +810,8 @@ int launch_vnl_sphere(sgpu_context *ctx, int ncol, const T *x, long ldx, T *out,
     constexpr int NCP = Mfma<T>::NCP, NC = Mfma<T>::NCA;
     const int max_np = 64;
     void *buf = nullptr;
-    SGPU_CHECK(sgpu_scratch(ctx, (size_t)nl.n_ent * max_np * ncol * sizeof(T) + 256, &buf));
+    SGPU_CHECK(sgpu_scratch(ctx, sgpu_vnl_scratch_bytes(ctx, sizeof(T) == 4 ? SGPU_F32 : SGPU_F64,
+                                                        ncol), &buf));
     T *part = (T *)buf;
     sgpu_tmark t;
     SGPU_CHECK(sgpu_tbegin(ctx, &t));
@@ -843,8 +844,8 @@ int launch_vnl_typed(sgpu_context *ctx, int ncol, const T *x, long ldx, T *out, 
     }
     void *buf = nullptr;
     const size_t part_elems = (size_t)nl.n_rows * ncol;
-    const size_t alpha_elems = (size_t)nl.n_alpha * ncol;
-    SGPU_CHECK(sgpu_scratch(ctx, (part_elems + alpha_elems) * sizeof(T) + 256, &buf));
+    SGPU_CHECK(sgpu_scratch(ctx, sgpu_vnl_scratch_bytes(ctx, sizeof(T) == 4 ? SGPU_F32 : SGPU_F64,
+                                                        ncol), &buf));
     T *part = (T *)buf;
     T *alpha_s = part + part_elems;
     constexpr int NCP = Mfma<T>::NCP, NC = Mfma<T>::NCA;
@@ -1035,6 +1036,7 @@ int build_sphere_layout(sgpu_context *ctx, int n_influence, const int *ndc, cons
     sgpu_nonlocal &nl = ctx->nl;
     hipStream_t s = ctx->stream;
     nl.mode = 1;
+    nl.stored = (long)chir.size();
     nl.n_infl = n_influence;
     nl.n_alpha = n_alpha;
     nl.n_ent = n_influence;
@@ -1388,6 +1390,7 @@ extern "C" int sgpu_set_nonlocal(sgpu_context *ctx, int n_influence, const int *
     nl.n_alpha = n_alpha;
     nl.n_bricks = nb;
     nl.n_rows = nrows;
+    nl.stored = nrows * kBP;
     nl.max_np = max_np;
     nl.dV = dV;
     SGPU_CHECK(upload(&nl.brick_org, borg.data(), borg.size(), s));
@@ -1416,6 +1419,26 @@ extern "C" int sgpu_set_nonlocal(sgpu_context *ctx, int n_influence, const int *
     SGPU_CHECK(upload(&nl.arow_src, arow_src.data(), arow_src.size(), s));
     SGPU_CHECK(upload(&nl.gamma, gamma, (size_t)n_alpha, s));
     SGPU_HIP(hipStreamSynchronize(s));
+    return SGPU_OK;
+}
+
+size_t sgpu_vnl_scratch_bytes(const sgpu_context *ctx, int dtype, int ncol)
+{
+    const sgpu_nonlocal &nl = ctx->nl;
+    const size_t s = sgpu_dtype_size(dtype);
+    if (nl.n_alpha == 0 || ncol <= 0) return 0;
+    if (nl.mode == 1) return (size_t)nl.n_ent * 64 * ncol * s + 256;  // partials, 64 rows per entry
+    return ((size_t)nl.n_rows * ncol + (size_t)nl.n_alpha * ncol) * s + 256;
+}
+
+extern "C" int sgpu_nonlocal_layout(const sgpu_context *ctx, int *mode, long *stored)
+{
+    if (!ctx || !mode || !stored) {
+        sgpu_set_error("sgpu_nonlocal_layout: bad arguments");
+        return SGPU_ERR_ARG;
+    }
+    *mode = ctx->nl.mode;
+    *stored = ctx->nl.n_alpha > 0 ? ctx->nl.stored : 0;
     return SGPU_OK;
 }
 

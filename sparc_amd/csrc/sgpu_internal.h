@@ -82,6 +82,7 @@ struct sgpu_nonlocal {
     // list and Chi, entries ordered by colour (no two entries of a colour
     // share a grid point), images of one atom found through atom_img
     int mode = 0;               // 0 brick layout, 1 sphere layout
+    long stored = 0;            // projector values per copy, zero padding included
     int n_ent = 0, n_atoms = 0, n_colors = 0, max_qpad = 0, max_npad = 0;
     std::vector<int> color_start;   // host: first entry of each colour (+ end)
     int *ent_info = nullptr;    // per entry: qpad, pos offset, np, npad, atom, chi offset (6 ints)
@@ -167,6 +168,7 @@ struct sgpu_context {
     std::vector<sgpu_unet *> unets;
 };
 
+
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X (linear
 // workgroup id modulo 8), and each XCD has its own L2.  xcd_remap turns the
 // linear id into a logical id such that consecutive logical ids run on the
@@ -200,6 +202,8 @@ int sgpu_launch_stencil(sgpu_context *ctx, int dtype, int ncol, double a, double
                         const void *z, long ldz, double s1, double s2, bool allow_tiled);
 // ---- nonlocal.hip ----
 void sgpu_free_nonlocal(sgpu_nonlocal &nl);
+// Scratch bytes sgpu_launch_vnl takes for ncol columns of dtype.
+size_t sgpu_vnl_scratch_bytes(const sgpu_context *ctx, int dtype, int ncol);
 // out += scale * Vnl x  (alpha computed from x); dtype of x/out.
 int sgpu_launch_vnl(sgpu_context *ctx, int dtype, int ncol, const void *x, long ldx, void *out,
                     long ldo, double scale);

@@ -138,6 +138,7 @@ _SIGS = {
     "sgpu_unet_harvest": (C.c_int, [_vp, _dp, _dp, C.c_double, C.c_int, C.c_int, C.c_int, _ip]),
     "sgpu_unet_set_shift": (C.c_int, [_vp, C.c_double]),
     "sgpu_nonlocal_integrals": (C.c_int, [_vp, _vp, _dp]),
+    "sgpu_nonlocal_layout": (C.c_int, [_vp, _ip, C.POINTER(C.c_long)]),
     "sgpu_local_forces": (C.c_int, [_vp, C.c_int, _vp, C.c_int, _ip, _ip, _ip, _dp, C.c_double, _dp,
                                     _dp, _dp, _dp, C.c_int, _dp, _dp]),
     "sgpu_unet_npairs": (C.c_int, [_vp]),
@@ -509,6 +510,13 @@ class Context:
             _ptr(centre), float(ref_cutoff), _ptr(_f64(phi).ravel()), _ptr(_f64(setup.Vc).ravel()),
             _ptr(bsum), _ptr(_f64(vxc).ravel()), n_atom, _ptr(floc), _ptr(fxc)))
         return floc, fxc
+
+    def nonlocal_layout(self):
+        """(mode, stored): 0 bricks / 1 spheres, and the projector values stored
+        per copy with zero padding (sgpu_nonlocal_layout)."""
+        mode, stored = C.c_int(), C.c_long()
+        self._check(self.lib.sgpu_nonlocal_layout(self.h, C.byref(mode), C.byref(stored)))
+        return mode.value, stored.value
 
     def nonlocal_integrals(self, psi: "Block") -> np.ndarray:
         """[4, n_alpha, ncol]: dV <Chi, psi> and <Chi, d psi / dx_d>
