@@ -20,8 +20,10 @@ C3  128^3 Poisson, 216 Gaussian charges, rel. tol 1e-8: Jacobi-AAR and
     U-Net run in fewer iterations.
 C4  216-atom Si/Al (10 % Al, Fermi-Dirac 0.01 Ha), 480 states: FP64, mixed
     and mixed + learned Poisson preconditioner SCFs converge; the mixed runs
-    within 1e-5 Ha/atom of FP64; atomic forces of the mixed runs within
-    1e-6 Ha/Bohr of FP64.
+    within 1e-5 Ha/atom of FP64; atomic forces of the mixed run within
+    1e-6 Ha/Bohr of FP64 and of the learned run within 1e-4 (its Poisson
+    solves stop at other iterates within TOL_POISSON); with every Poisson
+    solve converged to 1e-10, learned and Jacobi forces agree to 1e-6.
 
 The U-Net used by C3 and C4 is trained on the device, as in the benchmark,
 on (residual, error) pairs of Jacobi-AAR Poisson solves of the 64-atom
@@ -257,11 +259,17 @@ def test_c4_sial216_fp64_mixed_learned(unet_weights):
         if all(n % 4 == 0 for n in (g.nx, g.ny, g.nz)):
             net = ctx.unet(seed=1)
             net.set_weights(unet_weights)
-        plan = [("f64", gpu.F64, None), ("mixed", gpu.F32, None), ("learned", gpu.F32, net)]
-        for tag, dt, pn in plan:
+        # the *_tight runs solve every Poisson problem to 1e-10 instead of
+        # TOL_POISSON = TOL_SCF / 100
+        plan = [("f64", gpu.F64, None, None), ("mixed", gpu.F32, None, None),
+                ("learned", gpu.F32, net, None), ("mixed_tight", gpu.F32, None, 1e-10),
+                ("learned_tight", gpu.F32, net, 1e-10)]
+        for tag, dt, pn, tolp in plan:
             _step(f"C4 SCF {tag}")
-            r = ctx.scf_run(st.b, st.rho_at, dict(p, dtype=dt), rho_core=st.rho_core,
-                            poisson_net=pn, keep_psi=True)
+            pp = (dict(p, dtype=dt) if tolp is None
+                  else dict(p, dtype=dt, tol_poisson=tolp, maxit_poisson=10000))
+            r = ctx.scf_run(st.b, st.rho_at, pp, rho_core=st.rho_core, poisson_net=pn,
+                            keep_psi=True)
             occ = occupations(r["lambda_"], r["Efermi"], p["beta"], p["smear_type"])
             ints = ctx.nonlocal_integrals(r["psi"])
             fnl = F.nonlocal_forces_from_integrals(st, ints, occ, p["occfac"])
@@ -282,5 +290,18 @@ def test_c4_sial216_fp64_mixed_learned(unet_weights):
               f"{r['poisson_iters']} (FP64 {f['poisson_iters']}), fallbacks {r['poisson_fallbacks']}")
         assert r["converged"] == 1
         assert de < 1e-5
-        assert dF < 1e-6
+        # the mixed run follows the FP64 trajectory; the learned run's Poisson
+        # solves stop at other iterates within TOL_POISSON (2.2e-6 relative
+        # residual here), which moves the forces of an SCF converged to
+        # TOL_SCF = 2.2e-4 by up to ~1e-4 Ha/Bohr
+        assert dF < (1e-6 if tag == "mixed" else 1e-4)
+    # with the Poisson solves converged far below that, the learned
+    # preconditioner gives the forces of the Jacobi solves to 1e-6 Ha/Bohr
+    mt, lt = runs["mixed_tight"], runs["learned_tight"]
+    dFt = np.abs(lt["forces"] - mt["forces"]).max()
+    det = abs(lt["Etot"] - mt["Etot"]) / st.n_atom
+    print(f"C4 Poisson tol 1e-10: learned vs Jacobi max dF {dFt:.2e}, dE/atom {det:.2e}, "
+          f"Poisson AAR {lt['poisson_iters']} vs {mt['poisson_iters']}")
+    assert mt["converged"] == 1 and lt["converged"] == 1
+    assert dFt < 1e-6
     assert runs["learned"]["poisson_iters"] < f["poisson_iters"]
