@@ -191,6 +191,24 @@ int sgpu_lanczos(sgpu_context *ctx, const double *x0, double tol_min, double tol
 int sgpu_chefsi_dev(sgpu_context *ctx, sgpu_block *X, sgpu_block *Y, int m, double a, double b,
                     double a0, double *lambda);
 
+/* Bytes of the device scratch array sgpu_hamiltonian_dev_noalloc uses (one
+ * copy per GPU, held by the library until the process ends). */
+#define SGPU_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/* sgpu_hamiltonian_dev that allocates nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * stream capture, tight serving loops).  Same result as sgpu_hamiltonian_dev,
+ * bit for bit.  The nonlocal partials live in a file-scope device array of
+ * SGPU_NOALLOC_SCRATCH_BYTES per GPU instead of the context's scratch; they
+ * take (projector rows of the layout) x ncol x element size bytes, and a call
+ * that needs more is refused with SGPU_ERR_ARG (apply fewer columns per
+ * call).  The stencil uses the tile shape already chosen for this
+ * (precision, column count) by an earlier allocating call, else 32 x 16 x 2
+ * (every shape gives the same bits); event timers are not recorded.  All
+ * calls on one GPU share its array: calls that overlap on one GPU race. */
+int sgpu_hamiltonian_dev_noalloc(sgpu_context *ctx, double c, const sgpu_block *x,
+                                 sgpu_block *Hx);
+
 /* ---- self-consistent field loop ------------------------------------- */
 
 /* Exchange-correlation functionals (EXCHANGE_CORRELATION: LDA_PZ / LDA_PW) */

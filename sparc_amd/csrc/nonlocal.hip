@@ -802,16 +802,29 @@ __global__ __launch_bounds__(256) void nls_apply_proj(T *__restrict__ out, long 
     }
 }
 
+// The partials buffer: the caller's workspace when one is given (it must
+// hold sgpu_vnl_scratch_bytes), else the context's scratch.
+int vnl_partials(sgpu_context *ctx, int dtype, int ncol, void *ws, size_t ws_bytes, void **buf)
+{
+    const size_t need = sgpu_vnl_scratch_bytes(ctx, dtype, ncol);
+    if (!ws) return sgpu_scratch(ctx, need, buf);
+    if (ws_bytes < need) {
+        sgpu_set_error("nonlocal: %zu workspace bytes given, %zu needed", ws_bytes, need);
+        return SGPU_ERR_ARG;
+    }
+    *buf = ws;
+    return SGPU_OK;
+}
+
 template <typename T>
 int launch_vnl_sphere(sgpu_context *ctx, int ncol, const T *x, long ldx, T *out, long ldo,
-                      double scale, const T *chir, const T *chit)
+                      double scale, const T *chir, const T *chit, void *ws, size_t ws_bytes)
 {
     const sgpu_nonlocal &nl = ctx->nl;
     constexpr int NCP = Mfma<T>::NCP, NC = Mfma<T>::NCA;
     const int max_np = 64;
     void *buf = nullptr;
-    SGPU_CHECK(sgpu_scratch(ctx, sgpu_vnl_scratch_bytes(ctx, sizeof(T) == 4 ? SGPU_F32 : SGPU_F64,
-                                                        ncol), &buf));
+    SGPU_CHECK(vnl_partials(ctx, sizeof(T) == 4 ? SGPU_F32 : SGPU_F64, ncol, ws, ws_bytes, &buf));
     T *part = (T *)buf;
     sgpu_tmark t;
     SGPU_CHECK(sgpu_tbegin(ctx, &t));
@@ -834,7 +847,7 @@ int launch_vnl_sphere(sgpu_context *ctx, int ncol, const T *x, long ldx, T *out,
 
 template <typename T>
 int launch_vnl_typed(sgpu_context *ctx, int ncol, const T *x, long ldx, T *out, long ldo,
-                     double scale, const T *phi, const T *phic)
+                     double scale, const T *phi, const T *phic, void *ws, size_t ws_bytes)
 {
     const sgpu_nonlocal &nl = ctx->nl;
     if (nl.n_bricks == 0 || nl.n_alpha == 0 || ncol <= 0) return SGPU_OK;
@@ -844,8 +857,7 @@ int launch_vnl_typed(sgpu_context *ctx, int ncol, const T *x, long ldx, T *out, 
     }
     void *buf = nullptr;
     const size_t part_elems = (size_t)nl.n_rows * ncol;
-    SGPU_CHECK(sgpu_scratch(ctx, sgpu_vnl_scratch_bytes(ctx, sizeof(T) == 4 ? SGPU_F32 : SGPU_F64,
-                                                        ncol), &buf));
+    SGPU_CHECK(vnl_partials(ctx, sizeof(T) == 4 ? SGPU_F32 : SGPU_F64, ncol, ws, ws_bytes, &buf));
     T *part = (T *)buf;
     T *alpha_s = part + part_elems;
     constexpr int NCP = Mfma<T>::NCP, NC = Mfma<T>::NCA;
@@ -1443,21 +1455,21 @@ extern "C" int sgpu_nonlocal_layout(const sgpu_context *ctx, int *mode, long *st
 }
 
 int sgpu_launch_vnl(sgpu_context *ctx, int dtype, int ncol, const void *x, long ldx, void *out,
-                    long ldo, double scale)
+                    long ldo, double scale, void *ws, size_t ws_bytes)
 {
     if (ctx->nl.mode == 1) {
         if (ctx->nl.n_ent == 0 || ncol <= 0) return SGPU_OK;
         if (dtype == SGPU_F64)
             return launch_vnl_sphere<double>(ctx, ncol, (const double *)x, ldx, (double *)out, ldo,
-                                             scale, ctx->nl.chir64, ctx->nl.chit64);
+                                             scale, ctx->nl.chir64, ctx->nl.chit64, ws, ws_bytes);
         return launch_vnl_sphere<float>(ctx, ncol, (const float *)x, ldx, (float *)out, ldo, scale,
-                                        ctx->nl.chir32, ctx->nl.chit32);
+                                        ctx->nl.chir32, ctx->nl.chit32, ws, ws_bytes);
     }
     if (dtype == SGPU_F64)
         return launch_vnl_typed<double>(ctx, ncol, (const double *)x, ldx, (double *)out, ldo,
-                                        scale, ctx->nl.phi64, ctx->nl.phic64);
+                                        scale, ctx->nl.phi64, ctx->nl.phic64, ws, ws_bytes);
     return launch_vnl_typed<float>(ctx, ncol, (const float *)x, ldx, (float *)out, ldo, scale,
-                                   ctx->nl.phi32, ctx->nl.phic32);
+                                   ctx->nl.phi32, ctx->nl.phic32, ws, ws_bytes);
 }
 
 int sgpu_launch_vnl_chain(sgpu_context *ctx, int dtype, int ncol, const void *x, long ldx,

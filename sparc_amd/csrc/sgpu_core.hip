@@ -61,6 +61,7 @@ extern "C" int sgpu_create(sgpu_context **out, int device, int nx, int ny, int n
         return SGPU_ERR_ARG;
     }
     SGPU_HIP(hipSetDevice(device));
+    SGPU_CHECK(sgpu_noalloc_setup(device));
     sgpu_context *ctx = new sgpu_context();
     ctx->device = device;
     ctx->nx = nx; ctx->ny = ny; ctx->nz = nz;

@@ -197,16 +197,21 @@ int sgpu_timer_resolve(sgpu_context *ctx);
 
 // ---- kernel launchers (stencil.hip) ----
 // out = s1 * ((a Lap + bdiag diag(v) + c) x) + s2 * z, per column (z may be null).
+// may_tune = false never times tile shapes (the timing allocates a temporary
+// output): a shape already chosen for (dtype, ncol) is used, else 32x16x2.
 int sgpu_launch_stencil(sgpu_context *ctx, int dtype, int ncol, double a, double bdiag, double c,
                         const void *v, const void *x, long ldx, void *out, long ldo,
-                        const void *z, long ldz, double s1, double s2, bool allow_tiled);
+                        const void *z, long ldz, double s1, double s2, bool allow_tiled,
+                        bool may_tune = true);
 // ---- nonlocal.hip ----
 void sgpu_free_nonlocal(sgpu_nonlocal &nl);
 // Scratch bytes sgpu_launch_vnl takes for ncol columns of dtype.
 size_t sgpu_vnl_scratch_bytes(const sgpu_context *ctx, int dtype, int ncol);
-// out += scale * Vnl x  (alpha computed from x); dtype of x/out.
+// out += scale * Vnl x  (alpha computed from x); dtype of x/out.  The
+// partials go to ws (ws_bytes >= sgpu_vnl_scratch_bytes) when ws is given,
+// else to the context's scratch.
 int sgpu_launch_vnl(sgpu_context *ctx, int dtype, int ncol, const void *x, long ldx, void *out,
-                    long ldo, double scale);
+                    long ldo, double scale, void *ws = nullptr, size_t ws_bytes = 0);
 // The same for one application of a chain in which each out is the next x
 // (the Chebyshev recurrence).  have_part: the partials of x were left by the
 // previous call (its *projected was true, and x is that call's out,
@@ -216,6 +221,10 @@ int sgpu_launch_vnl(sgpu_context *ctx, int dtype, int ncol, const void *x, long 
 int sgpu_launch_vnl_chain(sgpu_context *ctx, int dtype, int ncol, const void *x, long ldx,
                           void *out, long ldo, double scale, bool have_part, bool want_next,
                           bool *projected);
+// ---- noalloc.hip ----
+// Look up (once per device, with that device current) the address of the
+// file-scope scratch array that sgpu_hamiltonian_dev_noalloc uses.
+int sgpu_noalloc_setup(int device);
 // ---- unet.hip ----
 // Release a network's device memory and detach it from its context (called
 // by sgpu_destroy; the handle then only accepts sgpu_unet_destroy).

@@ -697,7 +697,7 @@ int tune_tile(sgpu_context *ctx, StencilParams<T> p, int ncol, int *best_cfg)
 template <typename T>
 int launch_typed(sgpu_context *ctx, int ncol, double a, double bdiag, double c, const T *v,
                  const T *x, long ldx, T *out, long ldo, const T *z, long ldz, double s1,
-                 double s2, bool allow_tiled)
+                 double s2, bool allow_tiled, bool may_tune)
 {
     StencilParams<T> p;
     p.x = x; p.out = out; p.z = z; p.v = v;
@@ -727,7 +727,9 @@ int launch_typed(sgpu_context *ctx, int ncol, double a, double bdiag, double c, 
         if (cfg < 0) {
             const auto key = std::make_pair((int)sizeof(T), ncol);
             auto it = ctx->tile_choice.find(key);
-            if (it == ctx->tile_choice.end()) {
+            if (it == ctx->tile_choice.end() && !may_tune) {
+                cfg = 0;
+            } else if (it == ctx->tile_choice.end()) {
                 SGPU_CHECK(tune_tile<T>(ctx, p, ncol, &cfg));
                 ctx->tile_choice[key] = cfg;
             } else {
@@ -756,14 +758,15 @@ int launch_typed(sgpu_context *ctx, int ncol, double a, double bdiag, double c, 
 
 int sgpu_launch_stencil(sgpu_context *ctx, int dtype, int ncol, double a, double bdiag, double c,
                         const void *v, const void *x, long ldx, void *out, long ldo, const void *z,
-                        long ldz, double s1, double s2, bool allow_tiled)
+                        long ldz, double s1, double s2, bool allow_tiled, bool may_tune)
 {
     if (dtype == SGPU_F64)
         return launch_typed<double>(ctx, ncol, a, bdiag, c, (const double *)v, (const double *)x,
                                     ldx, (double *)out, ldo, (const double *)z, ldz, s1, s2,
-                                    allow_tiled);
+                                    allow_tiled, may_tune);
     return launch_typed<float>(ctx, ncol, a, bdiag, c, (const float *)v, (const float *)x, ldx,
-                               (float *)out, ldo, (const float *)z, ldz, s1, s2, allow_tiled);
+                               (float *)out, ldo, (const float *)z, ldz, s1, s2, allow_tiled,
+                               may_tune);
 }
 
 extern "C" int sgpu_stencil_tile(const sgpu_context *ctx, int dtype, int ncol, int *tx, int *ty,

@@ -119,6 +119,7 @@ _SIGS = {
                            C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, _ip, _dp]),
     "sgpu_lap_plus_diag_dev": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, _vp, _vp]),
     "sgpu_hamiltonian_dev": (C.c_int, [_vp, C.c_double, _vp, _vp]),
+    "sgpu_hamiltonian_dev_noalloc": (C.c_int, [_vp, C.c_double, _vp, _vp]),
     "sgpu_chebyshev_filter_dev": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_double, C.c_double,
                                             C.c_double]),
     "sgpu_project_dev": (C.c_int, [_vp, _vp, _vp, _dp, _dp]),
@@ -411,6 +412,11 @@ class Context:
 
     def hamiltonian_dev(self, c: float, x: Block, Hx: Block):
         self._check(self.lib.sgpu_hamiltonian_dev(self.h, c, x.h, Hx.h))
+
+    def hamiltonian_dev_noalloc(self, c: float, x: Block, Hx: Block):
+        """sgpu_hamiltonian_dev without any device allocation (the library's
+        fixed per-GPU scratch array holds the nonlocal partials)."""
+        self._check(self.lib.sgpu_hamiltonian_dev_noalloc(self.h, c, x.h, Hx.h))
 
     def chebyshev_filter_dev(self, X: Block, Y: Block, m: int, a: float, b: float, a0: float):
         self._check(self.lib.sgpu_chebyshev_filter_dev(self.h, X.h, Y.h, m, a, b, a0))
