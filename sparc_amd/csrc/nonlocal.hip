@@ -7,15 +7,24 @@
 // dense Phi_b (kBP points x np_b projector columns of every atom touching
 // it).  The apply is then three launches over bricks x column blocks:
 //
-//  1. nlb_project: part[brick rows][cols] = Phi_b' X_b on 16x16x4 MFMA.  The
-//     brick's slice of NC columns is staged in LDS (coalesced 64-byte
-//     x-rows), each wavefront takes a quarter of the brick's points for a
-//     64-row x NC output tile, and the four partial tiles are summed in a
-//     fixed order through LDS.
+//  1. nlb_project_rs: part[brick rows][cols] = Phi_b' X_b on 16x16x4 MFMA,
+//     64 (FP32) / 32 (FP64) columns per block, X streamed from memory
+//     straight into the MFMA operands (16-byte loads of four consecutive
+//     points per lane), each wavefront taking a quarter of the brick's
+//     points, the four partial tiles summed in a fixed order through LDS.
 //  2. nlb_reduce: alpha_s[r][c] = scale dV Gamma_r sum over the brick rows of
 //     projector r (fixed order: deterministic, no atomics).
-//  3. nlb_apply: out_b += Phi_b beta_b on 16x16x4 MFMA with beta_b (np_b x NC)
-//     gathered from alpha_s into LDS and Phi_b read column-major.
+//  3. nlb_apply_rs: out_b += Phi_b beta_b on 16x16x4 MFMA with beta_b
+//     (np_b x NC) gathered from alpha_s into LDS, computed with points as
+//     MFMA rows so that each lane's results are four consecutive points of
+//     one column (one 16-byte read-add-write).  Optionally it also forms the
+//     next Chebyshev step's partials from the updated values (the brick
+//     chain, SPARC_AMD_NL_CHAIN_BRICK=1).
+//
+// The LDS-staged forms (nlb_project, 16 columns per block; the transposed
+// nlb_apply, 32 columns) stay selectable with SPARC_AMD_NLB=lds.  On the
+// 96^3 x 512 FP32 operator case: 1.04 -> 0.74 ms (projection) and
+// 1.23 -> 1.19 ms (apply) per application.
 //
 // Bricks touch each x value once per launch, so the gather redundancy of the
 // reference (overlapping spheres read the same points once per atom) turns
@@ -44,7 +53,8 @@ struct Mfma<float> {
     using acc = f32x4;
     using vec = float4;              // 16-byte staging vector
     static constexpr int NCP = 16;   // project: columns per block
-    static constexpr int NCA = 32;   // apply: columns per block
+    static constexpr int NCA = 32;   // sphere apply: columns per block
+    static constexpr int NCB = 64;   // brick apply: columns per block
     __device__ static acc run(float a, float b, acc c)
     {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
@@ -57,6 +67,7 @@ struct Mfma<double> {
     using vec = double2;
     static constexpr int NCP = 16;
     static constexpr int NCA = 16;
+    static constexpr int NCB = 16;
     __device__ static acc run(double a, double b, acc c)
     {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -86,6 +97,161 @@ __device__ __forceinline__ long brick_point(int q, int x0, int y0, int z0, int n
     const int gx = x0 + qx, gy = y0 + qy, gz = z0 + qz;
     if (gx >= nx || gy >= ny || gz >= nz) return -1;
     return ((long)gz * ny + gy) * nx + gx;
+}
+
+// part[brick row j][col] = sum_q Phi_b[q][j] X[col][q] on 16x16x4 MFMA, X
+// streamed from global memory straight into the B operands (no LDS staging).
+// The K index (points) is assigned so that a lane's 16-byte loads feed four
+// MFMAs: a step covers one 16-point x-row of the brick, and in MFMA u of the
+// step lane group lk supplies point 4 lk + u, so each lane loads the four
+// consecutive points 4 lk .. 4 lk + 3 of its column li (one 16-byte access in
+// FP32, two in FP64) and the 16 lanes of a group read 16 columns' segments of
+// the row; the A operands (Phi_b rows of the same points) follow the same
+// assignment, which is also the one nlb_apply_rs's fused projection uses, so
+// both give the same bits.  NC columns per block share every Phi load (NC/16
+// MFMAs per load); the next step's loads are issued before the current
+// step's MFMAs.  The four waves split the 512 points and their partial tiles
+// are summed in a fixed order (3 -> 2 -> 1 -> 0).  VEC: 16-byte loads (nx a
+// multiple of 4, ldx of 16 / sizeof(T)); otherwise scalar loads with
+// per-point checks.
+template <typename T, int NC, bool VEC>
+__global__ __launch_bounds__(256) void nlb_project_rs(const T *__restrict__ X, long ldx, int ncol,
+                                                      int nx, int ny, int nz,
+                                                      const int *__restrict__ brick_org,
+                                                      const int *__restrict__ brick_np,
+                                                      const long *__restrict__ brick_row,
+                                                      const T *__restrict__ phi,
+                                                      T *__restrict__ part)
+{
+    using M = Mfma<T>;
+    using V = typename Vec16<T>::type;
+    constexpr int VW = 16 / sizeof(T);         // elements per 16-byte access
+    constexpr int NV = 4 / VW;                 // 16-byte accesses per 4 points
+    constexpr int NCT = NC / 16;
+    constexpr int NSTEP = kBP / kWaves / 16;   // 16-point steps per wave
+    __shared__ typename M::acc red[kRT * NCT][64];
+    const int lid = xcd_remap(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int b = lid / gridDim.x;
+    const int col0 = (lid % gridDim.x) * NC;
+    const int x0 = brick_org[3 * b], y0 = brick_org[3 * b + 1], z0 = brick_org[3 * b + 2];
+    const int np = brick_np[b];
+    const long row0 = brick_row[b];
+    const T *ph = phi + row0 * kBP;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int qw = wave * (kBP / kWaves) + 4 * lk;  // first point of this lane, step 0
+    // column offsets of my NCT columns (clamped to column 0 past ncol, zeroed after the load)
+    long coff[NCT];
+    bool cok[NCT];
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) {
+        const int col = col0 + ct * 16 + li;
+        cok[ct] = col < ncol;
+        coff[ct] = cok[ct] ? (long)col * ldx : 0L;
+    }
+    auto load_x = [&](int s, T (&xv)[NCT][4]) {
+        const int q = qw + s * 16;
+        if constexpr (VEC) {
+            // the four points share y and z and lie inside the grid along x
+            // together (x0 a multiple of 16, nx of VW)
+            const long pt = brick_point(q, x0, y0, z0, nx, ny, nz);
+#pragma unroll
+            for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+                for (int v = 0; v < NV; v++) {
+                    const V w = *reinterpret_cast<const V *>(X + coff[ct] + (pt >= 0 ? pt : 0L) + v * VW);
+#pragma unroll
+                    for (int e = 0; e < VW; e++)
+                        xv[ct][v * VW + e] = (pt >= 0 && cok[ct]) ? w[e] : T(0);
+                }
+        } else {
+            long pts[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) pts[u] = brick_point(q + u, x0, y0, z0, nx, ny, nz);
+#pragma unroll
+            for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    T v = X[coff[ct] + (pts[u] >= 0 ? pts[u] : 0L)];
+                    xv[ct][u] = (pts[u] >= 0 && cok[ct]) ? v : T(0);
+                }
+        }
+    };
+    for (int j0 = 0; j0 < np; j0 += 16 * kRT) {
+        typename M::acc acc[kRT][NCT];
+#pragma unroll
+        for (int rt = 0; rt < kRT; rt++)
+#pragma unroll
+            for (int ct = 0; ct < NCT; ct++) acc[rt][ct] = typename M::acc{0, 0, 0, 0};
+        // rows past np read row np - 1 (in range) and are never stored
+        int jr[kRT];
+#pragma unroll
+        for (int rt = 0; rt < kRT; rt++) jr[rt] = min(j0 + rt * 16 + li, np - 1);
+        auto load_phi = [&](int s, T (&av)[4][kRT]) {
+            const int q = qw + s * 16;
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int rt = 0; rt < kRT; rt++) av[u][rt] = ph[(long)(q + u) * np + jr[rt]];
+        };
+        T xc[NCT][4], xn[NCT][4];
+        T ac[4][kRT], an[4][kRT];
+        load_x(0, xc);
+        load_phi(0, ac);
+        // not unrolled: one step's loads in flight while the previous step's
+        // MFMAs run (a full unroll hoisted every load and spilled)
+#pragma unroll 1
+        for (int s = 0; s < NSTEP; s++) {
+            const int sn = min(s + 1, NSTEP - 1);  // the last step reloads itself (cache hits)
+            load_x(sn, xn);
+            load_phi(sn, an);
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int rt = 0; rt < kRT; rt++) {
+                    if (j0 + rt * 16 < np) {
+#pragma unroll
+                        for (int ct = 0; ct < NCT; ct++)
+                            acc[rt][ct] = M::run(ac[u][rt], xc[ct][u], acc[rt][ct]);
+                    }
+                }
+#pragma unroll
+            for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+                for (int u = 0; u < 4; u++) xc[ct][u] = xn[ct][u];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int rt = 0; rt < kRT; rt++) ac[u][rt] = an[u][rt];
+        }
+        // fixed-order cross-wave sum: wave 3 -> 2 -> 1 -> 0
+        for (int w = kWaves - 1; w >= 1; w--) {
+            if (wave == w)
+#pragma unroll
+                for (int rt = 0; rt < kRT; rt++)
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ct++) red[rt * NCT + ct][lane] = acc[rt][ct];
+            __syncthreads();
+            if (wave == w - 1)
+#pragma unroll
+                for (int rt = 0; rt < kRT; rt++)
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ct++) acc[rt][ct] += red[rt * NCT + ct][lane];
+            __syncthreads();
+        }
+        if (wave == 0) {
+#pragma unroll
+            for (int rt = 0; rt < kRT; rt++)
+#pragma unroll
+                for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int j = j0 + rt * 16 + M::drow(lk, r);
+                        const int col = col0 + ct * 16 + li;
+                        if (j < np && col < ncol) part[(row0 + j) * ncol + col] = acc[rt][ct][r];
+                    }
+        }
+    }
 }
 
 // part[brick row j][col] = sum_q Phi_b[q][j] X[col][q]  on 16x16x4 MFMA.
@@ -330,6 +496,221 @@ __global__ __launch_bounds__(256) void nlb_apply(T *__restrict__ out, long ldo, 
                 const int col = col0 + ct * 16 + M::drow(lk, r);
                 if (col < ncol) out[(long)col * ldo + pt] = ov[ct][r] + acc[ct][r];
             }
+        }
+    }
+}
+
+// out_b += Phi_b beta_b for NC columns, computed untransposed on 16x16x4 MFMA:
+// D (16 points x 16 columns) = Phi_b(points, j) beta_b(j, columns) over one
+// brick x-row, with the MFMA row of each point permuted (FP64: row
+// 4 (p % 4) + p / 4 for point p; FP32: the identity) so that the four D
+// registers of lane (li, lk) are the consecutive points 4 lk .. 4 lk + 3 of
+// column li.  The read-add-write of out is then one 16-byte (FP32) or two
+// 16-byte (FP64) accesses per lane and column tile, the 16 lanes of a group
+// covering 16 columns' 64-byte (128-byte) segments of the row, and the A
+// operand (Phi_b column-major, points contiguous) is a coalesced load shared
+// by the NC / 16 column tiles.
+// With PROJ the updated values, still in registers, are also the B operands
+// of the projection Phi_b' out (point 4 lk + u in MFMA u, as in
+// nlb_project_rs): the pass writes the partials of out for the next
+// application of a Chebyshev chain into part_next, all RTM row tiles
+// accumulated at once (np <= 16 RTM), the four waves' tiles summed in a fixed
+// order.  Every lane's loads of a row step are issued before its MFMAs.
+template <typename T, int NC, int RTM, bool PROJ>
+__global__ __launch_bounds__(256) void nlb_apply_rs(T *__restrict__ out, long ldo, int ncol, int nx,
+                                                    int ny, int nz, const int *__restrict__ brick_org,
+                                                    const int *__restrict__ brick_np,
+                                                    const long *__restrict__ brick_row,
+                                                    const int *__restrict__ row_alpha,
+                                                    const T *__restrict__ phi,
+                                                    const T *__restrict__ phic,
+                                                    const T *__restrict__ alpha_s,
+                                                    T *__restrict__ part_next)
+{
+    using M = Mfma<T>;
+    using V = typename Vec16<T>::type;
+    constexpr int VW = 16 / sizeof(T);        // elements per 16-byte access
+    constexpr int NV = 4 / VW;                // 16-byte accesses per 4 points
+    constexpr int NCT = NC / 16;
+    constexpr int LDB = NC + 64 / sizeof(T);  // LDS stride of beta rows (bank spread)
+    constexpr int PW = kBP / kWaves;          // points per wave
+    extern __shared__ unsigned char smem_raw[];
+    T *beta = reinterpret_cast<T *>(smem_raw);  // np x LDB
+    __shared__ typename M::acc red[PROJ ? RTM * NCT : 1][64];
+    const int lid = xcd_remap(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int b = lid / gridDim.x;
+    const int col0 = (lid % gridDim.x) * NC;
+    const int x0 = brick_org[3 * b], y0 = brick_org[3 * b + 1], z0 = brick_org[3 * b + 2];
+    const int np = brick_np[b];
+    const long row0 = brick_row[b];
+    // beta gathered four elements per thread at a time, every load issued
+    // before the stores (clamped indices: the tail repeats the last element)
+    for (int base = 0; base < np * NC; base += 4 * 256) {
+        int jj[4], cc[4], rr[4];
+        T vv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int e = min(base + (int)threadIdx.x + 256 * u, np * NC - 1);
+            jj[u] = e / NC;
+            cc[u] = e - jj[u] * NC;
+            rr[u] = row_alpha[row0 + jj[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const bool ok = rr[u] >= 0 && col0 + cc[u] < ncol;
+            vv[u] = alpha_s[ok ? (long)rr[u] * ncol + col0 + cc[u] : 0L];
+            if (!ok) vv[u] = T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) beta[jj[u] * LDB + cc[u]] = vv[u];
+    }
+    __syncthreads();
+    const T *pc = phic + row0 * kBP;
+    const T *ph = phi + row0 * kBP;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    // MFMA row li of a tile holds point prow (the D layout puts row i of
+    // lane group lk in register r at i = 4 lk + r (FP32) or lk + 4 r (FP64))
+    const int prow = sizeof(T) == 4 ? li : 4 * (li & 3) + (li >> 2);
+    long coff[NCT];
+    bool cok[NCT];
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) {
+        const int col = col0 + ct * 16 + li;
+        cok[ct] = col < ncol;
+        coff[ct] = cok[ct] ? (long)col * ldo : 0L;
+    }
+    typename M::acc pacc[PROJ ? RTM : 1][NCT];
+    if constexpr (PROJ) {
+#pragma unroll
+        for (int rt = 0; rt < RTM; rt++)
+#pragma unroll
+            for (int ct = 0; ct < NCT; ct++) pacc[rt][ct] = typename M::acc{0, 0, 0, 0};
+    }
+    int jr[PROJ ? RTM : 1];
+    if constexpr (PROJ) {
+#pragma unroll
+        for (int rt = 0; rt < RTM; rt++) jr[rt] = min(rt * 16 + li, np - 1);
+    }
+#pragma unroll 1
+    for (int t = 0; t < PW / 16; t++) {
+        const int q0 = wave * PW + t * 16;
+        // my four points 4 lk .. 4 lk + 3 of the row (all inside or all
+        // outside the grid along y / z; along x when nx is a multiple of 4)
+        long pts[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) pts[r] = brick_point(q0 + 4 * lk + r, x0, y0, z0, nx, ny, nz);
+        const bool vec = pts[0] >= 0 && pts[3] == pts[0] + 3 && ((ldo & (VW - 1)) == 0) &&
+                         (reinterpret_cast<uintptr_t>(out + pts[0]) & 15) == 0;
+        // current values of out, loaded before the MFMA chain
+        T ov[NCT][4];
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++) {
+            if (vec) {
+#pragma unroll
+                for (int v = 0; v < NV; v++) {
+                    const V w = *reinterpret_cast<const V *>(out + coff[ct] + pts[0] + v * VW);
+#pragma unroll
+                    for (int e = 0; e < VW; e++) ov[ct][v * VW + e] = w[e];
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; r++) ov[ct][r] = out[coff[ct] + (pts[r] >= 0 ? pts[r] : 0L)];
+            }
+        }
+        typename M::acc acc[NCT];
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++) acc[ct] = typename M::acc{0, 0, 0, 0};
+        const T *pcq = pc + q0 + prow;
+        for (int jb = 0; jb < np; jb += 32) {
+            T aq[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int j = min(jb + 4 * u + lk, np - 1);
+                aq[u] = pcq[(long)j * kBP];  // A (point, j)
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int j = jb + 4 * u;
+                if (j < np) {
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ct++)  // B (j, col)
+                        acc[ct] = M::run(aq[u], beta[(j + lk) * LDB + ct * 16 + li], acc[ct]);
+                }
+            }
+        }
+        // new values: D register r of lane (li, lk) is point 4 lk + r of column li
+        T y[NCT][4];
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) y[ct][r] = ov[ct][r] + acc[ct][r];
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++) {
+            if (!cok[ct]) continue;
+            if (vec) {
+#pragma unroll
+                for (int v = 0; v < NV; v++) {
+                    V w;
+#pragma unroll
+                    for (int e = 0; e < VW; e++) w[e] = y[ct][v * VW + e];
+                    *reinterpret_cast<V *>(out + coff[ct] + pts[0] + v * VW) = w;
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    if (pts[r] >= 0) out[coff[ct] + pts[r]] = y[ct][r];
+            }
+        }
+        if constexpr (PROJ) {
+            // points outside the grid or columns past ncol contribute zero
+#pragma unroll
+            for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    if (pts[r] < 0 || !cok[ct]) y[ct][r] = T(0);
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                T aph[RTM];
+#pragma unroll
+                for (int rt = 0; rt < RTM; rt++) aph[rt] = ph[(long)(q0 + 4 * lk + u) * np + jr[rt]];
+#pragma unroll
+                for (int rt = 0; rt < RTM; rt++) {
+                    if (rt * 16 < np) {
+#pragma unroll
+                        for (int ct = 0; ct < NCT; ct++)
+                            pacc[rt][ct] = M::run(aph[rt], y[ct][u], pacc[rt][ct]);
+                    }
+                }
+            }
+        }
+    }
+    if constexpr (PROJ) {
+        for (int w = kWaves - 1; w >= 1; w--) {
+            if (wave == w)
+#pragma unroll
+                for (int rt = 0; rt < RTM; rt++)
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ct++) red[rt * NCT + ct][lane] = pacc[rt][ct];
+            __syncthreads();
+            if (wave == w - 1)
+#pragma unroll
+                for (int rt = 0; rt < RTM; rt++)
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ct++) pacc[rt][ct] += red[rt * NCT + ct][lane];
+            __syncthreads();
+        }
+        if (wave == 0) {
+#pragma unroll
+            for (int rt = 0; rt < RTM; rt++)
+#pragma unroll
+                for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int j = rt * 16 + M::drow(lk, r);
+                        const int col = col0 + ct * 16 + li;
+                        if (j < np && col < ncol) part_next[(row0 + j) * ncol + col] = pacc[rt][ct][r];
+                    }
         }
     }
 }
@@ -802,6 +1183,97 @@ __global__ __launch_bounds__(256) void nls_apply_proj(T *__restrict__ out, long 
     }
 }
 
+// SPARC_AMD_NLB=lds selects the LDS-staged nlb_project and the transposed
+// nlb_apply instead of the register-streamed kernels (no brick chain).
+bool nlb_lds_kernels()
+{
+    static const bool v = [] {
+        const char *e = getenv("SPARC_AMD_NLB");
+        return e && !strcmp(e, "lds");
+    }();
+    return v;
+}
+
+// Rows of Phi_b the fused apply + projection accumulates at once.
+constexpr int kChainRTM = 7;
+
+// out += Phi_b beta_b (beta_b gathered from alpha_s); with part_next also the
+// partials Phi_b' out of the result.
+template <typename T>
+int launch_nlb_apply(sgpu_context *ctx, int ncol, T *out, long ldo, const T *phi, const T *phic,
+                     const T *alpha_s, T *part_next)
+{
+    const sgpu_nonlocal &nl = ctx->nl;
+    if (nlb_lds_kernels() && !part_next) {
+        constexpr int NC = Mfma<T>::NCA;
+        const size_t shmem = (size_t)nl.max_np * (NC * 3 / 2) * sizeof(T);
+        if (shmem > 64 * 1024) {
+            sgpu_set_error("nonlocal: %d projector rows in one brick exceed the LDS of the apply",
+                           nl.max_np);
+            return SGPU_ERR_ARG;
+        }
+        nlb_apply<T><<<dim3((ncol + NC - 1) / NC, nl.n_bricks), 256, shmem, ctx->stream>>>(
+            out, ldo, ncol, ctx->nx, ctx->ny, ctx->nz, nl.brick_org, nl.brick_np, nl.brick_row,
+            nl.row_alpha, phic, alpha_s);
+        return SGPU_OK;
+    }
+    constexpr int NC = Mfma<T>::NCB, LDB = NC + 64 / (int)sizeof(T);
+    const size_t shmem = (size_t)nl.max_np * LDB * sizeof(T);
+    if (shmem > 96 * 1024) {
+        sgpu_set_error("nonlocal: %d projector rows in one brick exceed the LDS of the apply",
+                       nl.max_np);
+        return SGPU_ERR_ARG;
+    }
+    const dim3 grid((ncol + NC - 1) / NC, nl.n_bricks);
+    auto go = [&](auto kern) {
+        if (shmem > 32 * 1024)
+            SGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+        kern<<<grid, 256, shmem, ctx->stream>>>(out, ldo, ncol, ctx->nx, ctx->ny, ctx->nz,
+                                                nl.brick_org, nl.brick_np, nl.brick_row,
+                                                nl.row_alpha, phi, phic, alpha_s, part_next);
+        return SGPU_OK;
+    };
+    if (part_next) {
+        if (nl.max_np > 16 * kChainRTM) {
+            sgpu_set_error("nonlocal: fused apply + projection limited to %d rows per brick",
+                           16 * kChainRTM);
+            return SGPU_ERR_ARG;
+        }
+        return go(nlb_apply_rs<T, NC, kChainRTM, true>);
+    }
+    return go(nlb_apply_rs<T, NC, 1, false>);
+}
+
+// part = Phi_b' X for every brick: nlb_project_rs with 64 (FP32) / 32 (FP64)
+// columns per block, 16-byte loads where the grid and X allow them.
+template <typename T>
+void launch_nlb_project(sgpu_context *ctx, int ncol, const T *x, long ldx, const T *phi, T *part)
+{
+    const sgpu_nonlocal &nl = ctx->nl;
+    const hipStream_t stream = ctx->stream;
+    if (nlb_lds_kernels()) {
+        constexpr int NCP = Mfma<T>::NCP;
+        nlb_project<T><<<dim3((ncol + NCP - 1) / NCP, nl.n_bricks), 256, 0, stream>>>(
+            x, ldx, ncol, ctx->nx, ctx->ny, ctx->nz, nl.brick_org, nl.brick_np, nl.brick_row, phi,
+            part);
+        return;
+    }
+    // 16-byte loads of four points: the points of a lane lie inside the grid
+    // together when nx is a multiple of 4
+    constexpr int NC = sizeof(T) == 4 ? 64 : 32, VW = 16 / (int)sizeof(T);
+    const bool vec = ctx->nx % 4 == 0 && ldx % VW == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0;
+    const dim3 grid((ncol + NC - 1) / NC, nl.n_bricks);
+    if (vec)
+        nlb_project_rs<T, NC, true><<<grid, 256, 0, stream>>>(
+            x, ldx, ncol, ctx->nx, ctx->ny, ctx->nz, nl.brick_org, nl.brick_np, nl.brick_row, phi,
+            part);
+    else
+        nlb_project_rs<T, NC, false><<<grid, 256, 0, stream>>>(
+            x, ldx, ncol, ctx->nx, ctx->ny, ctx->nz, nl.brick_org, nl.brick_np, nl.brick_row, phi,
+            part);
+}
+
 // The partials buffer: the caller's workspace when one is given (it must
 // hold sgpu_vnl_scratch_bytes), else the context's scratch.
 int vnl_partials(sgpu_context *ctx, int dtype, int ncol, void *ws, size_t ws_bytes, void **buf)
@@ -860,11 +1332,9 @@ int launch_vnl_typed(sgpu_context *ctx, int ncol, const T *x, long ldx, T *out, 
     SGPU_CHECK(vnl_partials(ctx, sizeof(T) == 4 ? SGPU_F32 : SGPU_F64, ncol, ws, ws_bytes, &buf));
     T *part = (T *)buf;
     T *alpha_s = part + part_elems;
-    constexpr int NCP = Mfma<T>::NCP, NC = Mfma<T>::NCA;
     sgpu_tmark t;
     SGPU_CHECK(sgpu_tbegin(ctx, &t));
-    nlb_project<T><<<dim3((ncol + NCP - 1) / NCP, nl.n_bricks), 256, 0, ctx->stream>>>(
-        x, ldx, ncol, ctx->nx, ctx->ny, ctx->nz, nl.brick_org, nl.brick_np, nl.brick_row, phi, part);
+    launch_nlb_project<T>(ctx, ncol, x, ldx, phi, part);
     SGPU_HIP(hipGetLastError());
     SGPU_CHECK(sgpu_tend(ctx, &t, "nl_project"));
     const long total = (long)nl.n_alpha * ncol;
@@ -872,10 +1342,7 @@ int launch_vnl_typed(sgpu_context *ctx, int ncol, const T *x, long ldx, T *out, 
         part, ncol, nl.n_alpha, nl.arow_ptr, nl.arow_src, nl.gamma, scale * nl.dV, alpha_s);
     SGPU_HIP(hipGetLastError());
     SGPU_CHECK(sgpu_tbegin(ctx, &t));
-    const size_t shmem = (size_t)nl.max_np * (NC * 3 / 2) * sizeof(T);
-    nlb_apply<T><<<dim3((ncol + NC - 1) / NC, nl.n_bricks), 256, shmem, ctx->stream>>>(
-        out, ldo, ncol, ctx->nx, ctx->ny, ctx->nz, nl.brick_org, nl.brick_np, nl.brick_row,
-        nl.row_alpha, phic, alpha_s);
+    SGPU_CHECK(launch_nlb_apply<T>(ctx, ncol, out, ldo, phi, phic, alpha_s, nullptr));
     SGPU_HIP(hipGetLastError());
     SGPU_CHECK(sgpu_tend(ctx, &t, "nl_apply"));
     return SGPU_OK;
@@ -955,6 +1422,63 @@ int launch_vnl_chain_sphere(sgpu_context *ctx, int ncol, const T *x, long ldx, T
         }
     }
     SGPU_CHECK(sgpu_tend(ctx, &t, "nl_apply"));
+    return SGPU_OK;
+}
+
+// One application of a Chebyshev chain with the brick layout: the partials of
+// x are those the previous application left (have_part) or a projection of x;
+// with want_next the apply also forms the partials of out (nlb_apply_rs with
+// PROJ), so the next application skips its projection and the re-read of its
+// input.  Partials ping-pong between two buffers in nl.chain, alpha_s after
+// them.
+template <typename T>
+int launch_vnl_chain_brick(sgpu_context *ctx, int ncol, const T *x, long ldx, T *out, long ldo,
+                           double scale, const T *phi, const T *phic, bool have_part,
+                           bool want_next, bool *projected)
+{
+    sgpu_nonlocal &nl = ctx->nl;
+    *projected = false;
+    if (nl.n_bricks == 0 || nl.n_alpha == 0 || ncol <= 0) return SGPU_OK;
+    if (nl.n_bricks > 65535) {
+        sgpu_set_error("nonlocal: %d bricks exceed the launch limit", nl.n_bricks);
+        return SGPU_ERR_ARG;
+    }
+    const bool fuse = want_next && ctx->nl_chain_brick && !nlb_lds_kernels() &&
+                      nl.max_np <= 16 * kChainRTM;
+    if (!fuse && !have_part)
+        return launch_vnl_typed<T>(ctx, ncol, x, ldx, out, ldo, scale, phi, phic, nullptr, 0);
+    const size_t part_elems = (size_t)nl.n_rows * ncol;
+    const size_t need = (2 * part_elems + (size_t)nl.n_alpha * ncol) * sizeof(T) + 256;
+    if (have_part && need > nl.chain_bytes) {
+        sgpu_set_error("nonlocal chain: partials of a different shape");
+        return SGPU_ERR_STATE;
+    }
+    void *buf = nullptr;
+    SGPU_CHECK(chain_buffer(ctx, need, &buf));
+    if (!have_part) nl.chain_cur = 0;
+    T *part = (T *)buf + (size_t)nl.chain_cur * part_elems;
+    T *part_next = (T *)buf + (size_t)(nl.chain_cur ^ 1) * part_elems;
+    T *alpha_s = (T *)buf + 2 * part_elems;
+    sgpu_tmark t;
+    if (!have_part) {
+        SGPU_CHECK(sgpu_tbegin(ctx, &t));
+        launch_nlb_project<T>(ctx, ncol, x, ldx, phi, part);
+        SGPU_HIP(hipGetLastError());
+        SGPU_CHECK(sgpu_tend(ctx, &t, "nl_project"));
+    }
+    const long total = (long)nl.n_alpha * ncol;
+    nlb_reduce<T><<<(unsigned)((total + 255) / 256), 256, 0, ctx->stream>>>(
+        part, ncol, nl.n_alpha, nl.arow_ptr, nl.arow_src, nl.gamma, scale * nl.dV, alpha_s);
+    SGPU_HIP(hipGetLastError());
+    SGPU_CHECK(sgpu_tbegin(ctx, &t));
+    SGPU_CHECK(launch_nlb_apply<T>(ctx, ncol, out, ldo, phi, phic, alpha_s,
+                                   fuse ? part_next : nullptr));
+    SGPU_HIP(hipGetLastError());
+    SGPU_CHECK(sgpu_tend(ctx, &t, "nl_apply"));
+    if (fuse) {
+        nl.chain_cur ^= 1;
+        *projected = true;
+    }
     return SGPU_OK;
 }
 
@@ -1174,9 +1698,8 @@ int project_alpha(sgpu_context *ctx, int ncol, const T *x, long ldx, double scal
         }
         SGPU_CHECK(sgpu_scratch(ctx, (size_t)nl.n_rows * ncol * sizeof(T) + 256, &buf));
         T *part = (T *)buf;
-        nlb_project<T><<<dim3((ncol + NCP - 1) / NCP, nl.n_bricks), 256, 0, ctx->stream>>>(
-            x, ldx, ncol, ctx->nx, ctx->ny, ctx->nz, nl.brick_org, nl.brick_np, nl.brick_row,
-            sizeof(T) == 8 ? (const T *)nl.phi64 : (const T *)nl.phi32, part);
+        launch_nlb_project<T>(ctx, ncol, x, ldx,
+                              sizeof(T) == 8 ? (const T *)nl.phi64 : (const T *)nl.phi32, part);
         SGPU_HIP(hipGetLastError());
         const long total = (long)nl.n_alpha * ncol;
         nlb_reduce<T><<<(unsigned)((total + 255) / 256), 256, 0, ctx->stream>>>(
@@ -1483,16 +2006,19 @@ int sgpu_launch_vnl_chain(sgpu_context *ctx, int dtype, int ncol, const void *x,
         return SGPU_ERR_STATE;
     }
     nl.chain_x = nullptr;
-    // brick layout: a fused apply + projection kept the projection's
-    // accumulators for up to 128 projector rows live across the brick and ran
-    // at 2 waves per SIMD, slower (3.25 ms per application on the 96^3 x 512
-    // operator case) than the two kernels (2.55 ms).  A second form (the
-    // apply for 16 columns writing the brick's final values to an LDS slice,
-    // then nlb_project's product on it) was also slower: 2.69 against
-    // 2.37 ms, since nlb_project is bound by its Phi loads and MFMA chain, not
-    // by staging X.  Only the sphere layout carries partials.
-    if (nl.mode != 1) return sgpu_launch_vnl(ctx, dtype, ncol, x, ldx, out, ldo, scale);
-    const int st =
+    int st;
+    if (nl.mode != 1) {
+        st = dtype == SGPU_F64
+                 ? launch_vnl_chain_brick<double>(ctx, ncol, (const double *)x, ldx, (double *)out,
+                                                  ldo, scale, nl.phi64, nl.phic64, have_part,
+                                                  want_next, projected)
+                 : launch_vnl_chain_brick<float>(ctx, ncol, (const float *)x, ldx, (float *)out, ldo,
+                                                 scale, nl.phi32, nl.phic32, have_part, want_next,
+                                                 projected);
+        if (st == SGPU_OK && *projected) nl.chain_x = out;
+        return st;
+    }
+    st =
         dtype == SGPU_F64
             ? launch_vnl_chain_sphere<double>(ctx, ncol, (const double *)x, ldx, (double *)out, ldo,
                                               scale, nl.chir64, nl.chit64, have_part, want_next,

@@ -132,6 +132,11 @@ struct sgpu_context {
     // fuse each filter step's nonlocal projection into the previous step's
     // apply (SPARC_AMD_NL_CHAIN=0 turns it off; results are bitwise equal)
     bool nl_chain = true;
+    // the brick layout's fused apply + next projection (nlb_apply_rs with the
+    // projection; SPARC_AMD_NL_CHAIN_BRICK=1 selects it): bitwise equal, but
+    // slower on the 96^3 x 512 operator case (2.23 ms per application against
+    // 1.19 + 0.74 ms for nlb_apply_rs and nlb_project_rs), so off by default
+    bool nl_chain_brick = false;
     // eigensolves of up to 180 states reduce to tridiagonal form in one
     // workgroup's LDS instead of rocsolver's dsytrd (SPARC_AMD_LDS_TRIDIAG=0
     // keeps dsyevd everywhere)

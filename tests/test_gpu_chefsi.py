@@ -216,6 +216,33 @@ def test_fp32_gram_accuracy(gpu, ns):
     assert np.max(np.abs(Mp - ref)) < 5e-6 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("ns", [300, 480])
+def test_fp32_project_rotate_many_states(gpu, oracle_mod, ns):
+    """At 256 states and more the FP32 Gram runs on the 128 x 128 BF16-piece
+    kernel (Mp: upper tiles only, mirrored) and the rotation on the BF16-piece
+    strip kernel.  Against FP64 products of the same FP32 inputs: Mp and Hp to
+    5e-6 / 1e-5 of their largest element, HY and Psi = Y Q to 2e-5 relative."""
+    from sparc_amd.workloads import Grid
+    g = Grid(20, 20, 20, 0.5, 0.5, 0.5)
+    rng = np.random.default_rng(ns + 1)
+    Y = rng.uniform(-0.5, 0.5, (g.nd, ns)) * np.exp(rng.uniform(-3, 3, ns))
+    Y32 = Y.astype(np.float32).astype(np.float64)
+    veff = rng.uniform(-1.0, 0.0, g.nd)
+    o = oracle_mod.Oracle(g, veff, None)
+    Q = np.linalg.qr(rng.standard_normal((ns, ns)))[0]
+    with gpu.Context(g) as ctx:
+        ctx.set_veff(veff)
+        Hp, Mp, HY = ctx.DP_Project_Hamiltonian(Y32, gpu.F32)
+        R = ctx.DP_Subspace_Rotation(Y32, Q, gpu.F32)
+    HYo = o.hamiltonian(Y32)
+    Mref = Y32.T @ Y32
+    Href = Y32.T @ HYo
+    assert np.max(np.abs(Mp - Mref)) < 5e-6 * np.max(np.abs(Mref))
+    assert np.max(np.abs(Hp - Href)) < 1e-5 * np.max(np.abs(Href))
+    assert rel_err(HY, HYo) < 2e-5
+    assert rel_err(R, Y32 @ Q) < 2e-5
+
+
 def _separate_sites_case(seed):
     """Four projector sites whose 1.6-Bohr spheres do not overlap (one
     colour in the sphere layout), one of them across the cell face and split
@@ -232,15 +259,17 @@ def _separate_sites_case(seed):
 
 @pytest.mark.parametrize("dtype", [0, 1])
 @pytest.mark.parametrize("overlap", [False, True])
-@pytest.mark.parametrize("mode", ["brick", "sphere"])
+@pytest.mark.parametrize("mode", ["brick", "brick_fused", "sphere"])
 def test_filter_nonlocal_chain(gpu, oracle_mod, monkeypatch, mode, overlap, dtype):
     """The filter forms each step's nonlocal partials inside the previous
-    step's apply.  Against the unfused path (SPARC_AMD_NL_CHAIN=0): bitwise
-    equal; against the oracle: the usual tolerance.  Both layouts; separate
-    spheres (one colour: the sphere layout fuses) and overlapping ones (the
-    sphere layout applies colour by colour, unfused)."""
+    step's apply (sphere layout with one colour; brick layout with
+    SPARC_AMD_NL_CHAIN_BRICK=1).  Against the unfused path
+    (SPARC_AMD_NL_CHAIN=0): bitwise equal; against the oracle: the usual
+    tolerance.  Separate spheres (one colour: the sphere layout fuses) and
+    overlapping ones (the sphere layout applies colour by colour, unfused)."""
     from tests.helpers import split_images
-    monkeypatch.setenv("SPARC_AMD_NL_MODE", mode)
+    monkeypatch.setenv("SPARC_AMD_NL_MODE", "sphere" if mode == "sphere" else "brick")
+    monkeypatch.setenv("SPARC_AMD_NL_CHAIN_BRICK", "1" if mode == "brick_fused" else "0")
     if overlap:
         g, veff, nl, rng = small_case(n=(16, 15, 14), n_sites=5, rc=2.6, seed=24, lmax=1)
         nl = split_images(nl, 1)
@@ -259,6 +288,8 @@ def test_filter_nonlocal_chain(gpu, oracle_mod, monkeypatch, mode, overlap, dtyp
             ctx.set_nonlocal(nl)
             bx = ctx.block(ncol, dtype).upload(X)
             by = ctx.block(ncol, dtype)
+            ctx.chebyshev_filter_dev(bx, by, 9, a, b, a0)  # first call times the stencil tiles
+            bx.upload(X)
             ctx.chebyshev_filter_dev(bx, by, 9, a, b, a0)
             out[chain] = by.download()
     assert np.array_equal(out["1"], out["0"])
