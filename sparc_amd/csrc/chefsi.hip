@@ -260,6 +260,272 @@ __global__ __launch_bounds__(256) void gram_bf16x3(const float *__restrict__ A, 
             }
 }
 
+// ---- FP32 subspace rotation from BF16 pieces on gfx950 matrix cores -------
+//
+// Psi = Y Q (DP_Subspace_Rotation, src/eigenSolver.c:1386-1439) for FP32
+// blocks: every value is split into three BF16 pieces (as in gram_bf16x3) and
+// the six products with i + j <= 2 are summed on v_mfma_f32_16x16x32_bf16
+// (16 x 16 output tile, K = 32 per instruction, FP32 accumulation).  One block
+// takes a 64-point strip of Y and produces all n output columns, so Y is read
+// and split once: the strip's 32-column K-chunks are staged in LDS as
+// [piece][point][k] BF16 (the A operands, 16-byte reads), while Q comes as
+// three BF16 planes prepared once per rotation (split_q_bf16x3), column-major
+// with K padded to a multiple of 32 (the B operands, 16-byte loads from L2).
+// The eight waves share the strip and take every eighth 16-column output tile.
+// Operand layout of the 16x16x32 instruction: A(i, k) in lane i + 16 (k / 8),
+// element k % 8; B(k, n) in lane n + 16 (k / 8), element k % 8; D(i, n) in
+// lane n + 16 (i / 4), register i % 4, so each lane's four results are four
+// consecutive points of one column (one 16-byte store).
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+constexpr int kRotPts = 64;        // points per block
+constexpr int kRotK = 32;          // K per chunk
+constexpr int kRotLd = kRotK + 8;  // LDS row (bf16): 80 bytes
+constexpr int kRotWaves = 8;       // waves per block
+constexpr int kRotMaxCT = 4;       // output column tiles per wave (n <= 512)
+
+__device__ __forceinline__ void split3(float v, unsigned short &p0, unsigned short &p1,
+                                       unsigned short &p2)
+{
+    p0 = bf16_bits(v);
+    const float r1 = v - bf16_val(p0);
+    p1 = bf16_bits(r1);
+    p2 = bf16_bits(r1 - bf16_val(p1));
+}
+
+// Q (n x n FP64, column-major) -> three BF16 planes qp[piece][c * nk + k],
+// zero for k >= n (nk = n rounded up to kRotK).
+__global__ void split_q_bf16x3(const double *__restrict__ q, int n, int nk,
+                               unsigned short *__restrict__ qp)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long tot = (long)n * nk;
+    if (t >= tot) return;
+    const int c = (int)(t / nk), k = (int)(t - (long)c * nk);
+    unsigned short p0 = 0, p1 = 0, p2 = 0;
+    if (k < n) split3((float)q[(long)c * n + k], p0, p1, p2);
+    qp[t] = p0;
+    qp[tot + t] = p1;
+    qp[2 * tot + t] = p2;
+}
+
+__global__ __launch_bounds__(64 * kRotWaves) void rot_bf16x3(const float *__restrict__ Y, long ldy,
+                                                  const unsigned short *__restrict__ qp, int n,
+                                                  int nk, long nd, float *__restrict__ out,
+                                                  long ldo)
+{
+    __shared__ __align__(16) unsigned short sa[3][kRotPts * kRotLd];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, g = lane >> 4;
+    const long p0 = (long)blockIdx.x * kRotPts;
+    const int ntile = (n + 15) / 16;
+    const long plane = (long)n * nk;
+    f32x4_t acc[4][kRotMaxCT];
+#pragma unroll
+    for (int pt = 0; pt < 4; pt++)
+#pragma unroll
+        for (int ct = 0; ct < kRotMaxCT; ct++) acc[pt][ct] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < nk; k0 += kRotK) {
+        // stage Y[p0 .. p0+63][k0 .. k0+31]: consecutive threads on
+        // consecutive points (256-byte column segments), all loads first
+        constexpr int NT = 64 * kRotWaves;
+        float v[kRotPts * kRotK / NT];
+#pragma unroll
+        for (int r = 0; r < kRotPts * kRotK / NT; r++) {
+            const int e = threadIdx.x + NT * r;
+            const int kk = e / kRotPts, pp = e % kRotPts;
+            const bool ok = k0 + kk < n && p0 + pp < nd;
+            v[r] = Y[ok ? (long)(k0 + kk) * ldy + p0 + pp : 0L];
+            if (!ok) v[r] = 0.f;
+        }
+        __syncthreads();  // the previous chunk's operands have been read
+#pragma unroll
+        for (int r = 0; r < kRotPts * kRotK / NT; r++) {
+            const int e = threadIdx.x + NT * r;
+            const int kk = e / kRotPts, pp = e % kRotPts;
+            unsigned short a0, a1, a2;
+            split3(v[r], a0, a1, a2);
+            sa[0][pp * kRotLd + kk] = a0;
+            sa[1][pp * kRotLd + kk] = a1;
+            sa[2][pp * kRotLd + kk] = a2;
+        }
+        __syncthreads();
+        bf16x8_t av[4][3];
+#pragma unroll
+        for (int pt = 0; pt < 4; pt++)
+#pragma unroll
+            for (int q = 0; q < 3; q++)
+                av[pt][q] = *reinterpret_cast<const bf16x8_t *>(&sa[q][(pt * 16 + li) * kRotLd + 8 * g]);
+#pragma unroll
+        for (int ct = 0; ct < kRotMaxCT; ct++) {
+            const int tile = wave + kRotWaves * ct;
+            if (tile >= ntile) break;
+            const int col = min(tile * 16 + li, n - 1);
+            bf16x8_t bv[3];
+#pragma unroll
+            for (int q = 0; q < 3; q++)
+                bv[q] = *reinterpret_cast<const bf16x8_t *>(qp + q * plane + (long)col * nk + k0 + 8 * g);
+#pragma unroll
+            for (int pt = 0; pt < 4; pt++) {
+                // smallest products first
+                f32x4_t c = acc[pt][ct];
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[pt][1], bv[1], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[pt][0], bv[2], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[pt][2], bv[0], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[pt][0], bv[1], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[pt][1], bv[0], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[pt][0], bv[0], c, 0, 0, 0);
+                acc[pt][ct] = c;
+            }
+        }
+    }
+#pragma unroll
+    for (int ct = 0; ct < kRotMaxCT; ct++) {
+        const int tile = wave + kRotWaves * ct;
+        if (tile >= ntile) break;
+        const int col = tile * 16 + li;
+        if (col >= n) continue;
+#pragma unroll
+        for (int pt = 0; pt < 4; pt++) {
+            const long p = p0 + pt * 16 + 4 * g;
+            float *o = out + (long)col * ldo + p;
+            if (p + 3 < nd && (reinterpret_cast<uintptr_t>(o) & 15) == 0) {
+                *reinterpret_cast<float4 *>(o) =
+                    make_float4(acc[pt][ct][0], acc[pt][ct][1], acc[pt][ct][2], acc[pt][ct][3]);
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    if (p + r < nd) o[r] = acc[pt][ct][r];
+            }
+        }
+    }
+}
+
+// FP32 Gram product C = A'B on v_mfma_f32_16x16x32_bf16 from three BF16
+// pieces per value (six products, as gram_bf16x3), for the large state counts
+// at which the per-block split of gram_bf16x3's 64 x 64 tiles lost to rocBLAS:
+// 128 x 128 output tiles (each staged value is split once per 128 output
+// columns), eight waves of 32 x 64, K staged 32 points at a time as
+// [piece][column][k] BF16 in LDS (16-byte operand reads).  SYM (A == B): only
+// the tiles on and above the diagonal are computed and each off-diagonal one
+// is also written transposed.  One block per (tile, K-slice); the slices are
+// summed in FP64 by sum_partials.
+constexpr int kG3T = 128;          // output tile
+constexpr int kG3K = 32;           // K per chunk
+constexpr int kG3Ld = kG3K + 8;    // LDS row (bf16): 80 bytes
+
+template <bool SYM>
+__global__ __launch_bounds__(512) void gram3_bf16(const float *__restrict__ A, long lda,
+                                                  const float *__restrict__ B, long ldb, int n,
+                                                  long nd, long kslice, float *__restrict__ part)
+{
+    __shared__ __align__(16) unsigned short sa[3][kG3T * kG3Ld], sb[3][kG3T * kG3Ld];
+    const int tiles = (n + kG3T - 1) / kG3T;
+    // the tiles of one K-slice are consecutive logical ids -> one XCD, so
+    // the slice's columns they share are L2 hits
+    const int lid = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int tid_blk = lid % gridDim.x;
+    int ti, tj;
+    if (SYM) {  // upper-triangle tile index -> (ti, tj), ti <= tj
+        int t = tid_blk;
+        ti = 0;
+        while (t >= tiles - ti) {
+            t -= tiles - ti;
+            ti++;
+        }
+        tj = ti + t;
+    } else {
+        ti = tid_blk % tiles;
+        tj = tid_blk / tiles;
+    }
+    const int slice = lid / gridDim.x;
+    const long k0 = (long)slice * kslice, k1 = min(nd, k0 + kslice);
+    const int i0 = ti * kG3T, j0 = tj * kG3T;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, g = lane >> 4;
+    const int wi = (wave >> 1) * 32, wj = (wave & 1) * 64;  // wave's 32 x 64 sub-tile
+    f32x4_t acc[2][4];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    // stage: element e = (column, k), k fastest (128-byte column segments);
+    // the next chunk's loads are issued before this chunk's MFMAs
+    constexpr int PER = kG3T * kG3K / 512;
+    float va[PER], vb[PER];
+    auto load = [&](long kb) {
+#pragma unroll
+        for (int r = 0; r < PER; r++) {
+            const int e = threadIdx.x + 512 * r;
+            const int kk = e % kG3K, c = e / kG3K;
+            const long k = kb + kk;
+            const bool oka = k < k1 && i0 + c < n, okb = k < k1 && j0 + c < n;
+            va[r] = A[oka ? (long)(i0 + c) * lda + k : 0L];
+            vb[r] = B[okb ? (long)(j0 + c) * ldb + k : 0L];
+            if (!oka) va[r] = 0.f;
+            if (!okb) vb[r] = 0.f;
+        }
+    };
+    load(k0);
+    for (long kb = k0; kb < k1; kb += kG3K) {
+        __syncthreads();  // the previous chunk's operands have been read
+#pragma unroll
+        for (int r = 0; r < PER; r++) {
+            const int e = threadIdx.x + 512 * r;
+            const int kk = e % kG3K, c = e / kG3K;
+            const int o = c * kG3Ld + kk;
+            unsigned short p0, p1, p2;
+            split3(va[r], p0, p1, p2);
+            sa[0][o] = p0;
+            sa[1][o] = p1;
+            sa[2][o] = p2;
+            split3(vb[r], p0, p1, p2);
+            sb[0][o] = p0;
+            sb[1][o] = p1;
+            sb[2][o] = p2;
+        }
+        __syncthreads();
+        if (kb + kG3K < k1) load(kb + kG3K);
+        bf16x8_t av[2][3], bv[4][3];
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int q = 0; q < 3; q++)
+                av[a][q] = *reinterpret_cast<const bf16x8_t *>(&sa[q][(wi + a * 16 + li) * kG3Ld + 8 * g]);
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+#pragma unroll
+            for (int q = 0; q < 3; q++)
+                bv[b][q] = *reinterpret_cast<const bf16x8_t *>(&sb[q][(wj + b * 16 + li) * kG3Ld + 8 * g]);
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                f32x4_t c = acc[a][b];
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][1], bv[b][1], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][0], bv[b][2], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][2], bv[b][0], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][0], bv[b][1], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][1], bv[b][0], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][0], bv[b][0], c, 0, 0, 0);
+                acc[a][b] = c;
+            }
+    }
+    float *pp = part + (long)slice * n * n;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int i = i0 + wi + a * 16 + 4 * g + r;
+                const int j = j0 + wj + b * 16 + li;
+                if (i < n && j < n) {
+                    pp[(long)j * n + i] = acc[a][b][r];
+                    if (SYM && ti != tj) pp[(long)i * n + j] = acc[a][b][r];
+                }
+            }
+}
+
 template <typename T>
 rocblas_status gemm_tn_batched(rocblas_handle h, int n, long k, const T *A, long lda, long sa,
                                const T *B, long ldb, long sb, T *C, long sc, int batch);
@@ -284,8 +550,19 @@ rocblas_status gemm_tn_batched<float>(rocblas_handle h, int n, long k, const flo
                                          (rocblas_int)ldb, sb, &zero, C, n, sc, batch);
 }
 
-// SPARC_AMD_GRAM=rocblas: rocBLAS for every FP32 Gram; =bf16x3: the BF16-piece
-// kernel for every FP32 Gram; unset: BF16 pieces below 256 columns.
+// SPARC_AMD_ROT=rocblas: rocBLAS sgemm for the FP32 rotation instead of rot_bf16x3.
+int rot_mode()
+{
+    static const int v = [] {
+        const char *e = getenv("SPARC_AMD_ROT");
+        return (e && !strcmp(e, "rocblas")) ? 1 : 0;
+    }();
+    return v;
+}
+
+// SPARC_AMD_GRAM=rocblas: rocBLAS for every FP32 Gram; =bf16x3: the 64 x 64
+// BF16-piece kernel for every FP32 Gram; unset: the 64 x 64 kernel below 256
+// columns, the 128 x 128 one (gram3_bf16) from 256.
 int gram_mode()
 {
     static const int v = [] {
@@ -297,9 +574,12 @@ int gram_mode()
     return v;
 }
 
+// symmetric: C is symmetric (Y'Y, or Y'HY with H symmetric); the 128 x 128
+// kernel then forms the tiles on and above the diagonal only and mirrors
+// them, which is what the reference's dsygvd reads of Hp and Mp ('U').
 template <typename T>
 int gram_tn_typed(sgpu_context *ctx, int n, long nd, const T *A, long lda, const T *B, long ldb,
-                  double *C)
+                  double *C, bool symmetric)
 {
     const long kslice = gram_kslice(n, nd, sizeof(T) == 4 ? SGPU_F32 : SGPU_F64);
     const int nslice = (int)((nd + kslice - 1) / kslice);
@@ -307,11 +587,27 @@ int gram_tn_typed(sgpu_context *ctx, int n, long nd, const T *A, long lda, const
     void *buf = nullptr;
     SGPU_CHECK(sgpu_scratch(ctx, (size_t)nslice * nn * sizeof(T), &buf));
     T *part = (T *)buf;
-    // BF16-piece kernel for Ns < 256; above that rocBLAS's FP32 GEMM was
-    // measured faster (512k x 480, 216-atom SCF: 6.1 vs 19.5 ms per iteration;
-    // the per-block split of every staged value into three pieces, not the
-    // re-reads of Y, bounds it: mapping a K-slice's tiles onto one XCD gave
-    // the same time)
+    // Below 256 states the 64 x 64 BF16-piece kernel; from 256 states the
+    // 128 x 128 one (512k x 480: 4.8 ms for Mp + Hp, rocBLAS sgemm 4.7 ms;
+    // the 64 x 64 kernel took 15 ms there, bound by splitting every staged
+    // value once per 64 output columns)
+    if (sizeof(T) == 4 && n >= 256 && gram_mode() == 0) {
+        // 128 x 128 tiles of the gfx950 BF16-piece kernel; Mp = Y'Y computes
+        // the upper triangle of tiles only
+        const int tiles = (n + kG3T - 1) / kG3T;
+        const bool sym = symmetric || (A == B && lda == ldb);
+        const unsigned nblk = sym ? (unsigned)(tiles * (tiles + 1) / 2) : (unsigned)(tiles * tiles);
+        if (sym)
+            gram3_bf16<true><<<dim3(nblk, nslice), 512, 0, ctx->stream>>>(
+                (const float *)A, lda, (const float *)B, ldb, n, nd, kslice, (float *)part);
+        else
+            gram3_bf16<false><<<dim3(nblk, nslice), 512, 0, ctx->stream>>>(
+                (const float *)A, lda, (const float *)B, ldb, n, nd, kslice, (float *)part);
+        SGPU_HIP(hipGetLastError());
+        sum_partials<T><<<(unsigned)((nn + 255) / 256), 256, 0, ctx->stream>>>(part, nslice, nn, C);
+        SGPU_HIP(hipGetLastError());
+        return SGPU_OK;
+    }
     if (sizeof(T) == 4 && ((n < 256 && gram_mode() == 0) || gram_mode() == 2)) {
         const int tilesn = (n + kGT - 1) / kGT;
         gram_bf16x3<<<dim3(tilesn * tilesn, nslice), 256, 0, ctx->stream>>>(
@@ -337,11 +633,13 @@ int gram_tn_typed(sgpu_context *ctx, int n, long nd, const T *A, long lda, const
 
 // C (n x n, FP64, device) = A' B over nd rows, split-K (FP64 sums of slices).
 int gram_tn(sgpu_context *ctx, int dtype, int n, long nd, const void *A, long lda, const void *B,
-            long ldb, double *C)
+            long ldb, double *C, bool symmetric)
 {
     if (dtype == SGPU_F64)
-        return gram_tn_typed<double>(ctx, n, nd, (const double *)A, lda, (const double *)B, ldb, C);
-    return gram_tn_typed<float>(ctx, n, nd, (const float *)A, lda, (const float *)B, ldb, C);
+        return gram_tn_typed<double>(ctx, n, nd, (const double *)A, lda, (const double *)B, ldb, C,
+                                     symmetric);
+    return gram_tn_typed<float>(ctx, n, nd, (const float *)A, lda, (const float *)B, ldb, C,
+                                symmetric);
 }
 
 // HY = H Y, then Mp = Y'Y at *dmat and Hp = Y'HY at *dmat + ncol^2 (FP64,
@@ -355,8 +653,8 @@ int project_device(sgpu_context *ctx, int dtype, int ncol, const void *Y, void *
     const long nn = (long)ncol * ncol;
     SGPU_CHECK(sgpu_reduce_buf(ctx, (size_t)(2 * nn), &dmat));
     SGPU_CHECK(sgpu_tbegin(ctx, &t));
-    SGPU_CHECK(gram_tn(ctx, dtype, ncol, ctx->nd, Y, ld, Y, ld, dmat));
-    SGPU_CHECK(gram_tn(ctx, dtype, ncol, ctx->nd, Y, ld, HY, ld, dmat + nn));
+    SGPU_CHECK(gram_tn(ctx, dtype, ncol, ctx->nd, Y, ld, Y, ld, dmat, true));
+    SGPU_CHECK(gram_tn(ctx, dtype, ncol, ctx->nd, Y, ld, HY, ld, dmat + nn, true));
     SGPU_CHECK(sgpu_tend(ctx, &t, "project_gemm"));
     *dmat_out = dmat;
     return SGPU_OK;
@@ -389,6 +687,17 @@ int rotate_device(sgpu_context *ctx, int dtype, int ns, const void *Y, long ldy,
         SGPU_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none,
                                 (rocblas_int)ctx->nd, ns, ns, &one, (const double *)Y,
                                 (rocblas_int)ldy, q64, ns, &zero, (double *)out, (rocblas_int)ldo));
+    } else if (rot_mode() != 1 && ns <= 16 * kRotWaves * kRotMaxCT) {
+        // BF16-piece rotation on matrix cores (SPARC_AMD_ROT=rocblas: rocBLAS sgemm)
+        const int nk = (ns + kRotK - 1) / kRotK * kRotK;
+        SGPU_CHECK(sgpu_scratch(ctx, 3 * (size_t)ns * nk * sizeof(unsigned short), &qd));
+        const long tot = (long)ns * nk;
+        split_q_bf16x3<<<(unsigned)((tot + 255) / 256), 256, 0, ctx->stream>>>(
+            q64, ns, nk, (unsigned short *)qd);
+        SGPU_HIP(hipGetLastError());
+        rot_bf16x3<<<(unsigned)((ctx->nd + kRotPts - 1) / kRotPts), 64 * kRotWaves, 0, ctx->stream>>>(
+            (const float *)Y, ldy, (const unsigned short *)qd, ns, nk, ctx->nd, (float *)out, ldo);
+        SGPU_HIP(hipGetLastError());
     } else {
         SGPU_CHECK(sgpu_scratch(ctx, nn * sizeof(float), &qd));
         SGPU_CHECK(sgpu_convert(ctx, q64, SGPU_F64, qd, SGPU_F32, nn));
