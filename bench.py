@@ -72,6 +72,34 @@ def mfma_entry(flops, ms, peak_key, issued_flops=None):
     return out
 
 
+def dense_entry(key, f32, nd, ns, ms):
+    """Matrix-core figures of the projection (Mp = Y'Y and Hp = Y'HY) or the
+    rotation (Y Q) at nd points and ns states, against the peak of the
+    arithmetic the kernel runs: the FP32 paths issue six BF16 products per
+    value pair (three pieces per value), the Gram from 256 states computes the
+    tiles on and above the diagonal only (128 x 128 tiles), FP64 is rocBLAS."""
+    nprod = 2 if key == "project_gemm" else 1
+    flops = nprod * 2.0 * nd * ns * ns
+    gram_env = os.environ.get("SPARC_AMD_GRAM", "")
+    rot_env = os.environ.get("SPARC_AMD_ROT", "")
+    if not f32:
+        e, kernel = mfma_entry(flops, ms, "fp64"), "rocblas dgemm"
+    elif key == "rotate_gemm" and rot_env != "rocblas" and ns <= 512:
+        nk = -(-ns // 32) * 32
+        e, kernel = mfma_entry(flops, ms, "bf16", 6 * 2.0 * nd * nk * ns), "rot_bf16x3"
+    elif key == "project_gemm" and gram_env != "rocblas" and (ns < 256 or gram_env == "bf16x3"):
+        t = -(-ns // 64)
+        e, kernel = mfma_entry(flops, ms, "bf16", 6 * 2 * 2.0 * nd * (64 * t) ** 2), "gram_bf16x3"
+    elif key == "project_gemm" and gram_env != "rocblas":
+        t = -(-ns // 128)
+        issued = 6 * 2 * 2.0 * nd * 128 ** 2 * (t * (t + 1) // 2)
+        e, kernel = mfma_entry(flops, ms, "bf16", issued), "gram3_bf16"
+    else:
+        e, kernel = mfma_entry(flops, ms, "fp32"), "rocblas sgemm"
+    e["kernel"] = kernel
+    return e
+
+
 def algorithmic_bytes(nd, ns, m, s, chi_elems):
     return s * nd * ns * (2 + 3 * (m - 1)) + m * s * nd + 2 * m * s * chi_elems
 
@@ -140,10 +168,13 @@ def nonlocal_dense(ctx, wl, rec, peak_key):
     chi, ns = int(wl.nloc.chi.size), wl.ncol
     out = {"layout": "bricks" if mode == 0 else "spheres", "stored_values": stored,
            "chi_values": chi}
+    names = {"nl_project": "nlb_project_rs" if mode == 0 else "nls_project",
+             "nl_apply": "nlb_apply_rs" if mode == 0 else "nls_apply / nls_apply_proj"}
     for key in ("nl_project", "nl_apply"):
         tot, cnt = rec[key]
         if cnt:
             out[key] = mfma_entry(2.0 * chi * ns, tot / cnt, peak_key, 2.0 * stored * ns)
+            out[key]["kernel"] = names[key]
     return out
 
 
@@ -247,20 +278,10 @@ def scf_bench(name, unet_weights=None, kerker_weights=None):
                     phases[k] = round(tot / r["scf_iters"], 3)
             dense = {}
             ns_, nd_ = p["nstates"], g.nd
-            pk = "fp64" if dt == gpu.F64 else "fp32"
-            gk = os.environ.get("SPARC_AMD_GRAM", "")
-            bf16_gram = dt == gpu.F32 and (gk == "bf16x3" or (gk != "rocblas" and ns_ < 256))
-            for key, nprod in (("project_gemm", 2), ("rotate_gemm", 1)):
+            for key in ("project_gemm", "rotate_gemm"):
                 tot, cnt = ctx.timer(key)
                 if cnt:
-                    flops = nprod * 2.0 * nd_ * ns_ * ns_
-                    if key == "project_gemm" and bf16_gram:
-                        e = mfma_entry(flops, tot / cnt, "bf16", 6 * flops)
-                        e["kernel"] = "gram_bf16x3"
-                    else:
-                        e = mfma_entry(flops, tot / cnt, pk)
-                        e["kernel"] = "rocblas"
-                    dense[key] = e
+                    dense[key] = dense_entry(key, dt == gpu.F32, nd_, ns_, tot / cnt)
             tot, cnt = ctx.timer("eigensolve")
             if cnt:
                 dense["eigensolve"] = {"ms": round(tot / cnt, 3), "n": ns_}
