@@ -208,8 +208,8 @@ def cpu_baseline(wl, ncol_sample=48, degree=None):
 def scf_bench(name, unet_weights=None, kerker_weights=None):
     """FP64 and mixed-precision device SCF of one psp8 system, plus the mixed
     run with the learned Poisson preconditioner when trained weights are
-    given and the grid allows it (dimensions multiples of 4, periodic), and
-    the same with a second network as the Kerker solves' preconditioner."""
+    given, and the same with a second network as the Kerker solves'
+    preconditioner."""
     from sparc_amd import gpu
     from sparc_amd.systems import scf_system
     t0 = time.perf_counter()
@@ -226,7 +226,7 @@ def scf_bench(name, unet_weights=None, kerker_weights=None):
                        + ("LDA_PW" if p["xc"] == 1 else "LDA_PZ") + ", "
                        + ("Fermi-Dirac" if p["smear_type"] == 0 else "Gaussian")
                        + f" smearing {1.0 / p['beta']:.4g} Ha, tol_scf {p['tol_scf']:.3g}"}
-    learned = unet_weights is not None and all(n % 4 == 0 for n in (g.nx, g.ny, g.nz))
+    learned = unet_weights is not None
     plan = [("f64", gpu.F64, False, False), ("mixed", gpu.F32, False, False)]
     if learned:
         plan.append(("mixed_learned", gpu.F32, True, False))
@@ -321,37 +321,17 @@ def scf_bench(name, unet_weights=None, kerker_weights=None):
 
 def train_unet(train_steps=400):
     """Train the learned Poisson preconditioner on the GPU, on (residual,
-    error) pairs harvested from Jacobi-AAR solves of the 64-atom Si SCF's
-    Poisson problems (52^3 grid; the 8-atom cell's 26^3 grid is not a
-    multiple of 4, which the U-Net's two halvings need).  Returns the FP32
-    weights and a record of the training."""
-    from sparc_amd import gpu
-    from sparc_amd.systems import scf_system
-    st, p = scf_system("si64")
-    with gpu.Context(st.grid) as ctx:
-        ctx.set_nonlocal(st.nloc)
-        rhos = [st.rho_at]
-        for k in (1, 2, 4):
-            r = ctx.scf_run(st.b, st.rho_at, dict(p, max_scf=k, dtype=gpu.F32), rho_core=st.rho_core)
-            rhos.append(r["rho"])
-        net = ctx.unet(seed=1)
-        th = time.perf_counter()
-        for rho in rhos:
-            net.harvest(4.0 * np.pi * (st.b + rho), np.zeros(st.grid.nd), tol=1e-8, max_iter=3000,
-                        stride=2, max_pairs=120)
-        th = time.perf_counter() - th
-        tt = time.perf_counter()
-        hist = net.train(train_steps, batch=4, lr=2e-3, seed=5)
-        tt = time.perf_counter() - tt
-        w = net.get_weights()
-        info = {"pairs": net.npairs, "steps": train_steps, "batch": 4,
-                "harvest_s": round(th, 2), "train_s": round(tt, 2),
-                "loss_first": float(hist[:5].mean()), "loss_last": float(hist[-20:].mean()),
-                "source": "Jacobi-AAR Poisson solves (tol 1e-8) of the si64 SCF's densities "
-                          "(guess and after 1, 2, 4 SCF iterations), 52^3"}
-        net.close()
-    print(f"[bench] unet: {info['pairs']} pairs, harvest {th:.1f}s, train {tt:.1f}s, "
-          f"loss {info['loss_first']:.3e} -> {info['loss_last']:.3e}", flush=True)
+    error) pairs harvested from Jacobi-AAR solves of the 8-atom (26^3, run on
+    28^3 by interpolation) and 64-atom (52^3) Si SCFs' Poisson problems, Adam
+    in blocks alternating between the two grids, the weights with the fewest
+    AAR iterations on a held-out 64-atom problem kept (sparc_amd.learned).
+    Returns the FP32 weights and a record of the training."""
+    from sparc_amd.learned import train_poisson_unet
+    w, info = train_poisson_unet(("si8", "si64"), steps=train_steps)
+    print(f"[bench] unet: pairs {info['pairs']}, harvest {info['harvest_s']:.1f}s, train "
+          f"{info['train_s']:.1f}s, loss {info['loss_first']:.3e} -> {info['loss_last']:.3e}, "
+          f"validation AAR iterations {info['validation_aar_iterations']}, kept after "
+          f"{info['kept_after_steps']} steps", flush=True)
     return w, info
 
 

@@ -327,7 +327,10 @@ int sgpu_scf_run(sgpu_context *ctx, const sgpu_scf_params *par, const double *b,
 
 /* A linear 3-level 3-D convolutional U-Net (16 channels, BF16 activations,
  * FP32 accumulation and master weights) bound to a context's periodic grid
- * (every dimension a multiple of 4).  Applied as f = (r + U(r)) / |D0| with
+ * (at least 8 points per axis; on a grid whose dimensions are not multiples
+ * of 4 the network runs on the grid rounded up to multiples of 4, input and
+ * output interpolated periodically-trilinearly).  Applied as
+ * f = (r + U(r)) / |D0| with
  * D0 = D2x[0] + D2y[0] + D2z[0]: the reference's Jacobi_preconditioner
  * (src/electrostatics.c:1682-1699) is the U = 0 case, which is also the
  * network's initial state.  No counterpart in the reference beyond that. */

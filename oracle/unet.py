@@ -22,6 +22,16 @@ Tensors are [C, Z, Y, X] (x fastest); weights W[co, ci, kz, ky, kx].
 conv stride s:   out[co, y] = sum W[co, ci, k] in[ci, s y + k - 1]
 transposed s=2:  out[co, x] = sum over (y, k) with 2 y + k - 1 = x of W[co, ci, k] in[ci, y]
 (indices periodic).
+
+Grids whose dimensions are not multiples of 4: the network runs on the
+internal grid M (each dimension rounded up to a multiple of 4) over the same
+periodic cell.  The residual is interpolated onto M and the network output
+back onto the grid, both by periodic trilinear interpolation (target point j
+of an axis sits at s = j n_src / n_dst source points and takes (1 - t, t) of
+source points floor(s), floor(s) + 1, t = s - floor(s)):
+    f = (r + I_{M->N} U(I_{N->M} r)) / |D0|,
+a fixed linear operator again; the backward pass takes dU through the
+transpose of I_{M->N}.
 """
 from __future__ import annotations
 
@@ -53,15 +63,41 @@ def split_weights(w: np.ndarray, C: int = 16):
     return ws
 
 
+def internal_shape(shape):
+    """The grid the network runs on: every dimension rounded up to a multiple of 4."""
+    return tuple(-(-n // 4) * 4 for n in shape)
+
+
+def interp_matrix(n_src: int, n_dst: int) -> np.ndarray:
+    """[n_dst, n_src] periodic linear interpolation along one axis."""
+    A = np.zeros((n_dst, n_src))
+    for j in range(n_dst):
+        i0, rem = divmod(j * n_src, n_dst)
+        t = rem / n_dst
+        A[j, i0] += 1.0 - t
+        A[j, (i0 + 1) % n_src] += t
+    return A
+
+
+def resample(x, shape_dst, transpose=False):
+    """Trilinear periodic interpolation of x [Z, Y, X] onto shape_dst; with
+    transpose, the transpose of the interpolation from shape_dst to x.shape."""
+    out = x
+    for ax in range(3):
+        n_src, n_dst = x.shape[ax], shape_dst[ax]
+        A = interp_matrix(n_dst, n_src).T if transpose else interp_matrix(n_src, n_dst)
+        out = np.moveaxis(np.tensordot(A, out, axes=([1], [ax])), 0, ax)
+    return out
+
+
 def _shift(a, dz, dy, dx):
     return np.roll(a, shift=(-dz, -dy, -dx), axis=(-3, -2, -1))
 
 
 def conv(x, W, stride):
-    """x [ci, n, n, n] -> [co, n/s, ...]."""
+    """x [ci, nz, ny, nx] -> [co, nz/s, ny/s, nx/s]."""
     co = W.shape[0]
-    n = x.shape[1] // stride
-    out = np.zeros((co,) + (n, n, n))
+    out = np.zeros((co,) + tuple(n // stride for n in x.shape[1:]))
     for kz in range(3):
         for ky in range(3):
             for kx in range(3):
@@ -73,14 +109,14 @@ def conv(x, W, stride):
 
 
 def convT(x, W):
-    """Transposed stride-2 conv: x [ci, n, n, n] -> [co, 2n, 2n, 2n]."""
+    """Transposed stride-2 conv: x [ci, nz, ny, nx] -> [co, 2nz, 2ny, 2nx]."""
     co = W.shape[0]
-    n = x.shape[1]
-    out = np.zeros((co, 2 * n, 2 * n, 2 * n))
+    fine = tuple(2 * n for n in x.shape[1:])
+    out = np.zeros((co,) + fine)
     for kz in range(3):
         for ky in range(3):
             for kx in range(3):
-                up = np.zeros((co, 2 * n, 2 * n, 2 * n))
+                up = np.zeros((co,) + fine)
                 up[:, ::2, ::2, ::2] = np.einsum("oc,czyx->ozyx", W[:, :, kz, ky, kx], x)
                 # value at fine 2y lands at fine 2y + k - 1
                 out += np.roll(up, shift=(kz - 1, ky - 1, kx - 1), axis=(1, 2, 3))
@@ -100,16 +136,17 @@ def conv_wgrad(x, g, stride, shape):
     return dW
 
 
-def conv_dgrad(g, W, stride, n_in):
+def conv_dgrad(g, W, stride):
     """dL/dx for out = conv(x, W, stride)."""
     ci = W.shape[1]
-    dx = np.zeros((ci, n_in, n_in, n_in))
+    shape_in = tuple(stride * n for n in g.shape[1:])
+    dx = np.zeros((ci,) + shape_in)
     for kz in range(3):
         for ky in range(3):
             for kx in range(3):
                 t = np.einsum("oc,ozyx->czyx", W[:, :, kz, ky, kx], g)
                 if stride == 2:
-                    up = np.zeros((ci, n_in, n_in, n_in))
+                    up = np.zeros((ci,) + shape_in)
                     up[:, ::2, ::2, ::2] = t
                     t = up
                 dx += np.roll(t, shift=(kz - 1, ky - 1, kx - 1), axis=(1, 2, 3))
@@ -139,9 +176,10 @@ def convT_dgrad(g, W):
 
 
 def forward(w, r, d0abs, C=16):
-    """r [n, n, n] -> (f, cache)."""
+    """r [nz, ny, nx] -> (f, cache)."""
     W = split_weights(w, C)
-    x0 = r[None]
+    m = internal_shape(r.shape)
+    x0 = (r if m == r.shape else resample(r, m))[None]
     a1 = conv(x0, W[0], 2)
     b1 = conv(a1, W[1], 1)
     a2 = conv(b1, W[2], 2)
@@ -149,31 +187,34 @@ def forward(w, r, d0abs, C=16):
     c2 = conv(b2, W[4], 1)
     s1 = convT(c2, W[5]) + b1
     c1 = conv(s1, W[6], 1)
-    U = convT(c1, W[7])[0]
+    Um = convT(c1, W[7])[0]
+    U = Um if m == r.shape else resample(Um, r.shape)
     f = (r + U) / d0abs
-    return f, dict(x0=x0, a1=a1, b1=b1, a2=a2, b2=b2, c2=c2, s1=s1, c1=c1, U=U)
+    return f, dict(x0=x0, a1=a1, b1=b1, a2=a2, b2=b2, c2=c2, s1=s1, c1=c1, U=U, shape=r.shape)
 
 
 def backward(w, cache, df, d0abs, C=16):
     """Gradient of a scalar loss w.r.t. the flat weights, given df = dL/df."""
     W = split_weights(w, C)
-    dU = (df / d0abs)[None]
+    m = cache["x0"].shape[1:]
+    dUn = df / d0abs
+    dU = (dUn if m == dUn.shape else resample(dUn, m, transpose=True))[None]
     g = [None] * 8
     g[7] = convT_wgrad(cache["c1"], dU, W[7].shape)
     dc1 = convT_dgrad(dU, W[7])
     g[6] = conv_wgrad(cache["s1"], dc1, 1, W[6].shape)
-    ds1 = conv_dgrad(dc1, W[6], 1, dc1.shape[1])
+    ds1 = conv_dgrad(dc1, W[6], 1)
     g[5] = convT_wgrad(cache["c2"], ds1, W[5].shape)
     dc2 = convT_dgrad(ds1, W[5])
     db1 = ds1.copy()
     g[4] = conv_wgrad(cache["b2"], dc2, 1, W[4].shape)
-    db2 = conv_dgrad(dc2, W[4], 1, dc2.shape[1])
+    db2 = conv_dgrad(dc2, W[4], 1)
     g[3] = conv_wgrad(cache["a2"], db2, 1, W[3].shape)
-    da2 = conv_dgrad(db2, W[3], 1, db2.shape[1])
+    da2 = conv_dgrad(db2, W[3], 1)
     g[2] = conv_wgrad(cache["b1"], da2, 2, W[2].shape)
-    db1 += conv_dgrad(da2, W[2], 2, 2 * da2.shape[1])
+    db1 += conv_dgrad(da2, W[2], 2)
     g[1] = conv_wgrad(cache["a1"], db1, 1, W[1].shape)
-    da1 = conv_dgrad(db1, W[1], 1, db1.shape[1])
+    da1 = conv_dgrad(db1, W[1], 1)
     g[0] = conv_wgrad(cache["x0"], da1, 2, W[0].shape)
     return np.concatenate([x.ravel() for x in g])
 

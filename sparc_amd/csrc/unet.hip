@@ -7,8 +7,12 @@
 //
 //     f = (r + U(r)) / |D0|,   D0 = D2x[0] + D2y[0] + D2z[0]
 //
-// Levels L0 = grid, L1 = grid / 2, L2 = grid / 4 (every grid dimension must be
-// a multiple of 4), C = 16 channels, periodic 3x3x3 kernels:
+// Levels L0 = grid, L1 = grid / 2, L2 = grid / 4, C = 16 channels, periodic
+// 3x3x3 kernels.  On a grid whose dimensions are not all multiples of 4 the
+// network runs on the internal grid M (each dimension rounded up to a
+// multiple of 4, same periodic cell): f = (r + I_{M->N} U(I_{N->M} r)) / |D0|
+// with periodic trilinear interpolation I (interp_trilinear, prolong_out,
+// and its transpose interp_adjoint in the backward pass).  Layers:
 //   R1 conv/2 1->C (L0->L1), B1 conv C->C, R2 conv/2 (L1->L2), B2, M2 conv,
 //   P2 transposed conv x2 (L2->L1) + skip from B1, C1 conv, P1 transposed
 //   conv x2 C->1 (L1->L0).
@@ -462,7 +466,7 @@ __global__ __launch_bounds__(256) void conv_1c(const TI *__restrict__ in, Dims d
 // so the eight 32-byte coarse voxels are loaded once for eight outputs
 // instead of once per tap of every output.  Taps summed z, y, x outer to
 // inner, channels in order.
-template <typename TI, typename TO>
+template <typename TI, typename TO, bool HAS_R = true>
 __global__ __launch_bounds__(256) void convT_c1_out_blk(const uint16_t *__restrict__ in, Dims di,
                                                         Dims dout, const float *__restrict__ W,
                                                         const TI *__restrict__ r, float out_scale,
@@ -489,7 +493,7 @@ __global__ __launch_bounds__(256) void convT_c1_out_blk(const uint16_t *__restri
 #pragma unroll
     for (int o = 0; o < 8; o++) {
         const int px = o & 1, py = (o >> 1) & 1, pz = o >> 2;
-        rv[o] = r[dout.vox(2 * cx + px, 2 * cy + py, 2 * cz + pz)];
+        rv[o] = HAS_R ? r[dout.vox(2 * cx + px, 2 * cy + py, 2 * cz + pz)] : TI(0);
     }
 #pragma unroll
     for (int o = 0; o < 8; o++) {
@@ -522,6 +526,113 @@ __global__ __launch_bounds__(256) void convT_c1_out_blk(const uint16_t *__restri
         f[dout.vox(2 * cx + px, 2 * cy + py, 2 * cz + pz)] =
             (TO)(((double)rv[o] + (double)u * out_scale) * inv_d0);
     }
+}
+
+// ---- grids whose dimensions are not multiples of 4 -------------------------
+// Periodic linear interpolation along one axis from n_src to n_dst points over
+// the same period: target point j sits at s = j n_src / n_dst source points and
+// takes (1 - t, t) of source points i0 = floor(s), i1 = i0 + 1 (mod n_src).
+__device__ __forceinline__ void lin_taps(int j, int n_src, int n_dst, int &i0, int &i1, float &t)
+{
+    const long num = (long)j * n_src;
+    i0 = (int)(num / n_dst);
+    t = (float)(num - (long)i0 * n_dst) / (float)n_dst;
+    i1 = i0 + 1 == n_src ? 0 : i0 + 1;
+}
+
+// out (FP32, grid dd) = trilinear periodic interpolation of in (grid ds).
+template <typename TI>
+__global__ __launch_bounds__(256) void interp_trilinear(const TI *__restrict__ in, Dims ds,
+                                                       float *__restrict__ out, Dims dd)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= dd.n()) return;
+    const int vq = (int)v / dd.nx;
+    const int x = (int)v - vq * dd.nx, y = vq % dd.ny, z = vq / dd.ny;
+    int ix[2], iy[2], iz[2];
+    float tx, ty, tz;
+    lin_taps(x, ds.nx, dd.nx, ix[0], ix[1], tx);
+    lin_taps(y, ds.ny, dd.ny, iy[0], iy[1], ty);
+    lin_taps(z, ds.nz, dd.nz, iz[0], iz[1], tz);
+    const float wx[2] = {1.f - tx, tx}, wy[2] = {1.f - ty, ty}, wz[2] = {1.f - tz, tz};
+    float acc = 0.f;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const int a = c >> 2, b = (c >> 1) & 1, e = c & 1;
+        acc += wz[a] * wy[b] * wx[e] * (float)in[ds.vox(ix[e], iy[b], iz[a])];
+    }
+    out[v] = acc;
+}
+
+// The preconditioner output on the grid from the network output um on the
+// internal grid: f = (r + I_{M->N} um) * inv_d0.
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void prolong_out(const float *__restrict__ um, Dims dm,
+                                                  const TI *__restrict__ r, Dims d0, double inv_d0,
+                                                  TO *__restrict__ f)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= d0.n()) return;
+    const int vq = (int)v / d0.nx;
+    const int x = (int)v - vq * d0.nx, y = vq % d0.ny, z = vq / d0.ny;
+    int ix[2], iy[2], iz[2];
+    float tx, ty, tz;
+    lin_taps(x, dm.nx, d0.nx, ix[0], ix[1], tx);
+    lin_taps(y, dm.ny, d0.ny, iy[0], iy[1], ty);
+    lin_taps(z, dm.nz, d0.nz, iz[0], iz[1], tz);
+    const float wx[2] = {1.f - tx, tx}, wy[2] = {1.f - ty, ty}, wz[2] = {1.f - tz, tz};
+    float u = 0.f;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const int a = c >> 2, b = (c >> 1) & 1, e = c & 1;
+        u += wz[a] * wy[b] * wx[e] * um[dm.vox(ix[e], iy[b], iz[a])];
+    }
+    f[v] = (TO)(((double)r[v] + (double)u) * inv_d0);
+}
+
+// Taps of the transpose of the interpolation from m to n points along one
+// axis: the points i of the n-grid whose interpolation reads point j of the
+// m-grid, with their weights (at most 5 candidates for n <= m; n >= 8 keeps
+// the wrapped candidates distinct).  Returns the count.
+__device__ __forceinline__ int adj_taps(int j, int n, int m, int *idx, float *w)
+{
+    int cnt = 0;
+    const int lo = (int)floor((double)(j - 1) * n / m), hi = (int)ceil((double)(j + 1) * n / m);
+    for (int ip = lo; ip <= hi && cnt < 6; ip++) {
+        const int i = ((ip % n) + n) % n;
+        int j0, j1;
+        float t;
+        lin_taps(i, m, n, j0, j1, t);
+        const float ww = (j0 == j ? 1.f - t : 0.f) + (j1 == j ? t : 0.f);
+        if (ww != 0.f) {
+            idx[cnt] = i;
+            w[cnt] = ww;
+            cnt++;
+        }
+    }
+    return cnt;
+}
+
+// out (internal grid dm) = transpose of the interpolation dm -> d0 applied to
+// g (grid d0): the gradient of the network output from that of the
+// preconditioner output.  Gather form, fixed order (no atomics).
+__global__ __launch_bounds__(256) void interp_adjoint(const float *__restrict__ g, Dims d0,
+                                                     float *__restrict__ out, Dims dm)
+{
+    const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= dm.n()) return;
+    const int vq = (int)v / dm.nx;
+    const int x = (int)v - vq * dm.nx, y = vq % dm.ny, z = vq / dm.ny;
+    int ix[6], iy[6], iz[6];
+    float wx[6], wy[6], wz[6];
+    const int nx = adj_taps(x, d0.nx, dm.nx, ix, wx);
+    const int ny = adj_taps(y, d0.ny, dm.ny, iy, wy);
+    const int nz = adj_taps(z, d0.nz, dm.nz, iz, wz);
+    float acc = 0.f;
+    for (int a = 0; a < nz; a++)
+        for (int b = 0; b < ny; b++)
+            for (int e = 0; e < nx; e++) acc += wz[a] * wy[b] * wx[e] * g[d0.vox(ix[e], iy[b], iz[a])];
+    out[v] = acc;
 }
 
 // Weight gradient G[a][b][k] = sum_y A[y][a] B[S y + k - 1][b] over the
@@ -771,7 +882,12 @@ void launch_convT_cc(const uint16_t *in, Dims di, uint16_t *out, Dims dout, cons
 
 struct sgpu_unet {
     sgpu_context *ctx = nullptr;
-    Dims d0{}, d1{}, d2{};
+    // d0: the context's grid; dm: the network's fine level (d0 with every
+    // dimension rounded up to a multiple of 4); d1 = dm / 2, d2 = dm / 4
+    Dims d0{}, dm{}, d1{}, d2{};
+    bool resample = false;         // dm != d0: interpolate in and out
+    float *rm = nullptr;           // the input interpolated onto dm
+    float *um = nullptr;           // the network output on dm / its gradient
     double d0sum = -1.0;  // D2x[0] + D2y[0] + D2z[0] (the Laplacian's diagonal)
     double d0abs = 1.0;
     double shift = 0.0;   // c of the operator Lap + c harvested for and trained on
@@ -828,15 +944,27 @@ int forward(sgpu_unet *u, const TI *r, TO *f, double out_scale)
         u->wb_current = true;
     }
     auto wb = [&](int l) -> const uint16_t * { return u->wb + (long)(l - 1) * kWcc; };
-    conv_1c<TI><<<nblk(u->d1.n()), 256, 0, s>>>(r, u->d0, u->a1, u->d1, layer_w(u, W, 0), 1.f);
+    if (u->resample) {
+        interp_trilinear<TI><<<nblk(u->dm.n()), 256, 0, s>>>(r, u->d0, u->rm, u->dm);
+        conv_1c<float><<<nblk(u->d1.n()), 256, 0, s>>>(u->rm, u->dm, u->a1, u->d1, layer_w(u, W, 0),
+                                                       1.f);
+    } else {
+        conv_1c<TI><<<nblk(u->d1.n()), 256, 0, s>>>(r, u->d0, u->a1, u->d1, layer_w(u, W, 0), 1.f);
+    }
     launch_conv_cc<1>(u->a1, u->d1, u->b1, u->d1, layer_w(u, W, 1), wb(1), nullptr, s);
     launch_conv_cc<2>(u->b1, u->d1, u->a2, u->d2, layer_w(u, W, 2), wb(2), nullptr, s);
     launch_conv_cc<1>(u->a2, u->d2, u->b2, u->d2, layer_w(u, W, 3), wb(3), nullptr, s);
     launch_conv_cc<1>(u->b2, u->d2, u->c2, u->d2, layer_w(u, W, 4), wb(4), nullptr, s);
     launch_convT_cc(u->c2, u->d2, u->s1, u->d1, layer_w(u, W, 5), wb(5), u->b1, s);
     launch_conv_cc<1>(u->s1, u->d1, u->c1, u->d1, layer_w(u, W, 6), wb(6), nullptr, s);
-    convT_c1_out_blk<TI, TO><<<nblk(u->d1.n()), 256, 0, s>>>(u->c1, u->d1, u->d0, layer_w(u, W, 7), r,
-                                                         1.f, out_scale, f);
+    if (u->resample) {
+        convT_c1_out_blk<float, float, false><<<nblk(u->d1.n()), 256, 0, s>>>(
+            u->c1, u->d1, u->dm, layer_w(u, W, 7), nullptr, 1.f, 1.0, u->um);
+        prolong_out<TI, TO><<<nblk(u->d0.n()), 256, 0, s>>>(u->um, u->dm, r, u->d0, out_scale, f);
+    } else {
+        convT_c1_out_blk<TI, TO><<<nblk(u->d1.n()), 256, 0, s>>>(u->c1, u->d1, u->d0, layer_w(u, W, 7),
+                                                             r, 1.f, out_scale, f);
+    }
     SGPU_HIP(hipGetLastError());
     return SGPU_OK;
 }
@@ -892,15 +1020,20 @@ int backward(sgpu_unet *u, const float *r, const float *df)
         wt = u->wt + (long)l * kWcc;
         wtb = u->wtb + (long)l * kWcc;
     };
-    // f = (r + U) / |D0 + shift|  ->  dU = df / |D0 + shift|, applied by loss_grad
-    float *dU = u->f32buf;  // overwrite df copy with dU
+    // f = (r + U) / |D0 + shift|  ->  dU = df / |D0 + shift|, applied by loss_grad;
+    // on a resampled grid the network output's gradient is I_{M->N}' dU
+    (void)df;
+    const float *dU = u->f32buf;
+    if (u->resample) {
+        interp_adjoint<<<nblk(u->dm.n()), 256, 0, s>>>(u->f32buf, u->d0, u->um, u->dm);
+        dU = u->um;
+    }
     // P1 (convT C->1, L1->L0): dW7[0][ci][k] = sum_y c1[y][ci] dU[2y+k-1]
     //   -> G[a=ci][b=0][k], transposed to [co=0][ci][k]
-    (void)df;
-    SGPU_CHECK(wgrad_launch(u, 2, u->c1, u->d1, dU, 1, u->d0, 1, 7));
+    SGPU_CHECK(wgrad_launch(u, 2, u->c1, u->d1, dU, 1, u->dm, 1, 7));
     // dc1[ci][y] = sum_k W7[0][ci][k] dU[2y+k-1]: a 1->C stride-2 conv with W'[ci][0][k]
     transform(7, 1);
-    conv_1c<float><<<nblk(u->d1.n()), 256, 0, s>>>(dU, u->d0, u->ga, u->d1, wt, 1.f);
+    conv_1c<float><<<nblk(u->d1.n()), 256, 0, s>>>(dU, u->dm, u->ga, u->d1, wt, 1.f);
     // C1 (conv C->C, L1): dW6 = sum_y dc1[y][co] s1[y+k-1][ci]
     SGPU_CHECK(wgrad_launch(u, 1, u->ga, u->d1, u->s1, kC, u->d1, 0, 6));
     transform(6, 0);
@@ -927,7 +1060,8 @@ int backward(sgpu_unet *u, const float *r, const float *df)
     transform(1, 0);
     launch_conv_cc<1>(u->ga, u->d1, u->gb, u->d1, wt, wtb, nullptr, s);  // da1
     // R1 (conv/2 1->C, L0->L1): dW0[co][0][k] = sum_y da1[y][co] r[2y+k-1]
-    SGPU_CHECK(wgrad_launch(u, 2, u->gb, u->d1, r, 1, u->d0, 0, 0));
+    // (the interpolated input of this pair's forward pass on a resampled grid)
+    SGPU_CHECK(wgrad_launch(u, 2, u->gb, u->d1, u->resample ? u->rm : r, 1, u->dm, 0, 0));
     SGPU_HIP(hipGetLastError());
     return SGPU_OK;
 }
@@ -965,9 +1099,10 @@ extern "C" int sgpu_unet_create(sgpu_context *ctx, int channels, unsigned long l
         sgpu_set_error("sgpu_unet_create: bad arguments (channels must be %d)", kC);
         return SGPU_ERR_ARG;
     }
-    if (ctx->nx % 4 || ctx->ny % 4 || ctx->nz % 4 || ctx->bcx || ctx->bcy || ctx->bcz) {
-        sgpu_set_error("sgpu_unet_create: needs a periodic grid with every dimension a multiple "
-                       "of 4 (got %d x %d x %d)", ctx->nx, ctx->ny, ctx->nz);
+    if (ctx->bcx || ctx->bcy || ctx->bcz || ctx->nx < 8 || ctx->ny < 8 || ctx->nz < 8) {
+        sgpu_set_error("sgpu_unet_create: needs a periodic grid of at least 8 points per axis "
+                       "(got %d x %d x %d%s)", ctx->nx, ctx->ny, ctx->nz,
+                       (ctx->bcx || ctx->bcy || ctx->bcz) ? ", Dirichlet" : "");
         return SGPU_ERR_ARG;
     }
     if (ctx->nd >= (1L << 31)) {  // the kernels index a level's voxels with int
@@ -984,8 +1119,10 @@ extern "C" int sgpu_unet_create(sgpu_context *ctx, int channels, unsigned long l
     sgpu_unet *u = new sgpu_unet();
     u->ctx = ctx;
     u->d0 = Dims{ctx->nx, ctx->ny, ctx->nz};
-    u->d1 = Dims{ctx->nx / 2, ctx->ny / 2, ctx->nz / 2};
-    u->d2 = Dims{ctx->nx / 4, ctx->ny / 4, ctx->nz / 4};
+    u->dm = Dims{(ctx->nx + 3) / 4 * 4, (ctx->ny + 3) / 4 * 4, (ctx->nz + 3) / 4 * 4};
+    u->resample = u->dm.nx != ctx->nx || u->dm.ny != ctx->ny || u->dm.nz != ctx->nz;
+    u->d1 = Dims{u->dm.nx / 2, u->dm.ny / 2, u->dm.nz / 2};
+    u->d2 = Dims{u->dm.nx / 4, u->dm.ny / 4, u->dm.nz / 4};
     u->d0sum = ctx->d2x[0] + ctx->d2y[0] + ctx->d2z[0];
     u->d0abs = std::fabs(u->d0sum);
     for (int l = 0; l < kLayers; l++) u->off[l + 1] = u->off[l] + layer_size(l);
@@ -1005,6 +1142,9 @@ extern "C" int sgpu_unet_create(sgpu_context *ctx, int channels, unsigned long l
     for (auto p : l2)
         if ((st = dalloc(u, p, n2))) return fail(st);
     if ((st = dalloc(u, &u->f32buf, (size_t)u->d0.n())) || (st = dalloc(u, &u->lpart, 256)))
+        return fail(st);
+    if (u->resample &&
+        ((st = dalloc(u, &u->rm, (size_t)u->dm.n())) || (st = dalloc(u, &u->um, (size_t)u->dm.n()))))
         return fail(st);
     u->partial_elems = (size_t)((u->d1.n() + kWChunk - 1) / kWChunk) * kC * kC * 27;
     if ((st = dalloc(u, &u->partial, u->partial_elems))) return fail(st);

@@ -562,6 +562,28 @@ class Context:
         return t.value, n.value
 
 
+def train_multigrid(nets, steps: int, batch: int = 4, lr: float = 2e-3, seed: int = 7,
+                    block: int = 50):
+    """Adam over pairs harvested on several grids (one network per grid, each
+    holding its own pairs): blocks of `block` steps on each network in turn,
+    the weights carried from one network to the next (each keeps its own Adam
+    moments).  Returns the loss of every step in order and the final weights,
+    which every network then holds."""
+    w = nets[0].get_weights()
+    hist, done, k = [], 0, 0
+    while done < steps:
+        n = min(block, steps - done)
+        net = nets[k % len(nets)]
+        net.set_weights(w)
+        hist.append(net.train(n, batch=batch, lr=lr, seed=seed + k))
+        w = net.get_weights()
+        done += n
+        k += 1
+    for net in nets:
+        net.set_weights(w)
+    return np.concatenate(hist), w
+
+
 class UNet:
     """The learned AAR preconditioner (sgpu_unet_*) bound to one context."""
 

@@ -5,12 +5,12 @@
   LDA_PW, Gaussian smearing 0.2 eV, TOL_SCF 1e-6, TOL_PSEUDOCHARGE 1e-9,
   24 states, degree 23) -- its reference output holds the energies.
 * ``si8`` / ``si64`` / ``sial216``: diamond-cubic silicon, a = 10.26 Bohr per
-  cubic cell, 1 / 2^3 / 3^3 cells on 26^3 / 52^3 / 80^3 grids (h = 0.3946 /
-  0.3946 / 0.3848), 24 / 160 / 480 states, LDA_PZ, seeded displacements
-  (0.02 / 0.02 / 0.03 Bohr); sial216 has 10 % seeded Al substitutions and
-  Fermi-Dirac smearing 0.01 Ha, and its grid dimensions are multiples of 4
-  so that the learned Poisson preconditioner (two grid halvings) applies.
-  ``si8u``: the 8-atom cell on a 28^3 grid, for the same reason.
+  cubic cell, 1 / 2^3 / 3^3 cells on 26^3 / 52^3 / 78^3 grids (h = 0.3946),
+  24 / 160 / 480 states, LDA_PZ, seeded displacements (0.02 / 0.02 /
+  0.03 Bohr); sial216 has 10 % seeded Al substitutions and Fermi-Dirac
+  smearing 0.01 Ha.  ``sial216_80``: the 216-atom cell on 80^3 (h = 0.3848),
+  ``si8u``: the 8-atom cell on 28^3 (grids whose dimensions are multiples of
+  4, on which the learned preconditioner runs without interpolation).
   Pseudopotentials: the ONCV psp8 files of SPARC's psps/ directory.
 """
 from __future__ import annotations
@@ -104,5 +104,7 @@ def scf_system(name: str):
     if name == "si8u":
         return silicon(1, 28, 24)
     if name == "sial216":
+        return silicon(3, 78, 480, disp=0.03, al_frac=0.1, fermi_dirac=0.01)
+    if name == "sial216_80":
         return silicon(3, 80, 480, disp=0.03, al_frac=0.1, fermi_dirac=0.01)
     raise ValueError(f"unknown SCF system {name!r}")

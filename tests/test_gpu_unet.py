@@ -28,12 +28,26 @@ def grid16():
     return Grid(16, 16, 16, 0.4, 0.4, 0.4)
 
 
-def test_unet_rejects_grid_not_multiple_of_4():
+def test_unet_rejects_dirichlet_grid():
     from sparc_amd import gpu
     from sparc_amd.workloads import Grid
-    with gpu.Context(Grid(26, 26, 26, 0.4, 0.4, 0.4)) as ctx:
-        with pytest.raises(gpu.SparcGPUError):
+    with gpu.Context(Grid(16, 16, 16, 0.4, 0.4, 0.4, bcz=1)) as ctx:
+        with pytest.raises(gpu.SparcGPUError, match="periodic"):
             ctx.unet()
+
+
+def test_unet_initial_is_jacobi_on_resampled_grid():
+    """On a grid that is not a multiple of 4 the network runs on the grid
+    rounded up (interpolating in and out); the zero network is still exactly
+    the Jacobi preconditioner."""
+    from sparc_amd import gpu
+    from sparc_amd.workloads import Grid
+    g = Grid(26, 22, 13, 0.4, 0.4, 0.4)
+    rng = np.random.default_rng(3)
+    r = rng.standard_normal(g.nd)
+    with gpu.Context(g) as ctx:
+        f = ctx.unet(seed=5).apply(r)
+    assert np.max(np.abs(f - r / _d0abs(g))) < 1e-14 * np.max(np.abs(r))
 
 
 def test_unet_initial_is_jacobi(grid16):
@@ -67,12 +81,18 @@ def test_unet_matrix_core_and_fma_layers_agree(grid16, monkeypatch):
     assert np.linalg.norm(g1 - g0) < 2e-2 * np.linalg.norm(g1)
 
 
-def test_unet_forward_and_gradient_match_oracle(grid16):
+@pytest.mark.parametrize("shape", [(16, 16, 16), (14, 13, 10)])
+def test_unet_forward_and_gradient_match_oracle(shape):
+    """Forward and weight gradient against the oracle; (14, 13, 10) runs the
+    network on 16 x 16 x 12 with the interpolations in and out (and the
+    transpose of the output interpolation in the backward pass)."""
     from oracle import unet as U
     from sparc_amd import gpu
+    from sparc_amd.workloads import Grid
+    grid16 = Grid(*shape, 0.4, 0.4, 0.4)
     rng = np.random.default_rng(4)
     d0 = _d0abs(grid16)
-    n = grid16.nx
+    zyx = (grid16.nz, grid16.ny, grid16.nx)
     with gpu.Context(grid16) as ctx:
         net = ctx.unet(seed=9)
         w = (rng.standard_normal(net.nparams) * 0.15).astype(np.float32)
@@ -83,13 +103,13 @@ def test_unet_forward_and_gradient_match_oracle(grid16):
         E = rng.standard_normal((2, grid16.nd)) * 0.05
         loss, g = net.loss_grad(R, E)
     wq = _bf16(w)
-    fo, _ = U.forward(wq, _bf16(r).reshape(n, n, n), d0)
+    fo, _ = U.forward(wq, _bf16(r).reshape(zyx), d0)
     # the Jacobi part r/|D0| is exact; compare the network part U
     u_gpu = f * d0 - r
     u_ref = fo.ravel() * d0 - _bf16(r)
     assert np.linalg.norm(u_gpu - u_ref) < 2e-2 * np.linalg.norm(u_ref)
-    Ro = [_bf16(x).reshape(n, n, n) for x in R]
-    Eo = [x.reshape(n, n, n) for x in E.astype(np.float32).astype(np.float64)]
+    Ro = [_bf16(x).reshape(zyx) for x in R]
+    Eo = [x.reshape(zyx) for x in E.astype(np.float32).astype(np.float64)]
     lo, go = U.loss_and_grad(wq, Ro, Eo, d0)
     assert abs(loss - lo) < 2e-2 * lo
     cos = float(g @ go) / (np.linalg.norm(g) * np.linalg.norm(go))
@@ -103,15 +123,17 @@ def test_unet_forward_and_gradient_match_oracle(grid16):
         off += ci * co * 27
 
 
-def test_unet_trained_preconditioner_beats_jacobi():
+@pytest.mark.parametrize("n", [32, 30])
+def test_unet_trained_preconditioner_beats_jacobi(n):
     """Train on pairs harvested from Jacobi-AAR Poisson solves on a 32^3
     cell, then solve a different right-hand side on the same grid: the same
     solution (tolerance 1e-8 relative residual for both) in fewer AAR
-    iterations."""
+    iterations.  n = 30 is not a multiple of 4: the network runs on 32^3 and
+    is trained through the interpolations."""
     from sparc_amd import gpu
     from sparc_amd.model import gaussian_density
     from sparc_amd.workloads import Grid
-    g = Grid(32, 32, 32, 0.4, 0.4, 0.4)
+    g = Grid(n, n, n, 12.8 / n, 12.8 / n, 12.8 / n)
     rng = np.random.default_rng(11)
 
     def rhs(seed):
