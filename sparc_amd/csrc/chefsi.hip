@@ -858,19 +858,380 @@ __global__ __launch_bounds__(kTridThreads) void sytd2_lower_lds(double *__restri
     }
 }
 
+// The same reduction (dsytd2, UPLO = 'L', the same D, E, tau and vectors)
+// for matrices too large for one workgroup's LDS, in one cooperative launch
+// of G workgroups.  Row r of the symmetric matrix lives in the LDS of
+// workgroup r mod G (both triangles, so the product A22 v is row-local).
+// Step i needs one grid barrier: before it every workgroup publishes its rows
+// of p = A22 v with its share of p'v, and the owner of row i+1 publishes that
+// row as it stands; after it every workgroup forms w = tau p - (tau^2 / 2)
+// (p'v) v and the next column x = row(i+1) - v(0) w - w(0) v itself, in the
+// same order (so every workgroup holds the same bits), computes the next
+// Householder vector from x and, in one pass over its own rows, applies the
+// rank-2 update and forms the next step's p.  Published values are written
+// and read with agent-scope atomics (coherent across the XCDs' L2s).  The
+// barrier: a release fence per wave, then each workgroup stores its step
+// number in its own mark (no read-modify-write on a shared counter) and one
+// wave polls all marks, then one acquire fence.  A poll that exceeds its time
+// limit sets an abort flag that every other workgroup sees in its own polls,
+// so the grid always drains.
+constexpr int kCoopMaxN = 1536;
+constexpr int kCoopMaxG = 128;
+
+struct CoopSync {
+    unsigned int abort;               // set by a poll that timed out
+    unsigned int pad[63];
+    unsigned int step[kCoopMaxG];     // last step each workgroup has published
+};
+
+__device__ __forceinline__ void coop_store(double *p, double v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double coop_load(const double *p)
+{
+    return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned int coop_load_u(unsigned int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid barrier: this workgroup marks step `target` published, then waits
+// until every workgroup has; false if the grid aborted.  The first wave polls
+// the G marks, one or two per lane.
+__device__ __forceinline__ bool coop_barrier(CoopSync *sy, unsigned int target, long tick_limit,
+                                             int *flag)
+{
+    const int G = gridDim.x, ln = threadIdx.x & 63;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");   // each wave's stores performed
+    __syncthreads();
+    if (threadIdx.x == 0)
+        __hip_atomic_store(&sy->step[blockIdx.x], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x < 64) {
+        const long t0 = wall_clock64();
+        int ok = 1;
+        for (;;) {
+            const bool r0 = ln >= G || coop_load_u(&sy->step[ln]) >= target;
+            const bool r1 = ln + 64 >= G || coop_load_u(&sy->step[ln + 64]) >= target;
+            if (__all(r0 && r1)) break;
+            if (coop_load_u(&sy->abort)) {
+                ok = 0;
+                break;
+            }
+            if (wall_clock64() - t0 > tick_limit) {
+                if (ln == 0)
+                    __hip_atomic_store(&sy->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (ln == 0) *flag = ok;
+    }
+    __syncthreads();
+    return *flag != 0;
+}
+
+// Sum over the workgroup in a fixed order, valid in every thread.
+template <int NT>
+__device__ __forceinline__ double coop_block_sum(double v, double *red)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int k = 0; k < NT / 64; k++) s += red[k];
+    return s;
+}
+
+// Householder vector of x(0:mm) (dlarfg): vv(0) = 1, vv(1:) = x(1:) /
+// (alpha - beta); tau = 0 (and vv(1:) = x(1:)) when x(1:) = 0.
+template <int NT>
+__device__ __forceinline__ void coop_householder(const double *x, int mm, double *vv,
+                                                 double *red, double &ti, double &beta)
+{
+    double ss = 0.0;
+    for (int j = 1 + (int)threadIdx.x; j < mm; j += NT) ss += x[j] * x[j];
+    const double xn2 = coop_block_sum<NT>(ss, red);
+    const double alpha = x[0];
+    double scal = 1.0;
+    if (xn2 == 0.0) {
+        ti = 0.0;
+        beta = alpha;
+    } else {
+        beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+        ti = (beta - alpha) / beta;
+        scal = 1.0 / (alpha - beta);
+    }
+    for (int j = threadIdx.x; j < mm; j += NT) vv[j] = j == 0 ? 1.0 : (ti != 0.0 ? x[j] * scal : x[j]);
+    __syncthreads();
+}
+
+// pub: two buffers of (p[n], row[n], dots[kCoopMaxG]) doubles, by step parity.
+template <int NT>
+__global__ __launch_bounds__(NT) void sytrd_lower_coop(double *__restrict__ A, int lda, int n,
+                                                       double *__restrict__ D,
+                                                       double *__restrict__ E,
+                                                       double *__restrict__ tau,
+                                                       double *__restrict__ pub,
+                                                       CoopSync *__restrict__ sy, long tick_limit)
+{
+    constexpr int NW = NT / 64;
+    extern __shared__ double sm[];
+    const int G = gridDim.x, g = blockIdx.x, t = threadIdx.x;
+    const int wv = t >> 6, ln = t & 63;
+    const int R = (n - 1 - g) / G + 1;      // rows g, g + G, ... below n
+    const int ldr = n + 1;
+    double *Ar = sm;                        // R rows of n (+1 pad)
+    double *x = Ar + (size_t)R * ldr;       // next column below the diagonal (n)
+    double *v = x + n;                      // Householder vector of this step (n)
+    double *vn = v + n;                     // ... of the next step (n)
+    double *w = vn + n;                     // w (n)
+    double *pown = w + n;                   // p of the own rows, by local index (R)
+    double *red = pown + R;                 // block-sum slots (NW)
+    double *wdot = red + NW;                // per-wave shares of p'v (NW)
+    __shared__ int flag;
+    const size_t pubsz = 2 * (size_t)n + kCoopMaxG;
+    for (int k = 0; k < R; k++) {
+        const int r = g + k * G;
+        for (int c = t; c < n; c += NT)
+            Ar[(size_t)k * ldr + c] = r >= c ? A[(long)c * lda + r] : A[(long)r * lda + c];
+    }
+    for (int j = t; j < n - 1; j += NT) x[j] = A[1 + j];
+    double dcur = A[0];                     // A(i, i) of the current step
+    double ti, beta;
+    __syncthreads();
+    coop_householder<NT>(x, n - 1, v, red, ti, beta);
+    // p = A22 v for step 0 (rows 1..n-1), and this workgroup's share of p'v
+    {
+        double dsum = 0.0;
+        for (int k = (g == 0 ? 1 : 0) + wv; k < R; k += NW) {
+            const double *row = Ar + (size_t)k * ldr + 1;
+            double s = 0.0;
+            for (int c = ln; c < n - 1; c += 64) s += row[c] * v[c];
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+            if (ln == 0) pown[k] = s;
+            dsum += s * v[g + k * G - 1];
+        }
+        if (ln == 0) wdot[wv] = dsum;
+    }
+    __syncthreads();
+    if (t == 0) {
+        double s = 0.0;
+        for (int q = 0; q < NW; q++) s += wdot[q];
+        coop_store(pub + 2 * n + g, s);
+    }
+    for (int i = 0; i + 1 < n; i++) {
+        const int m = n - i - 1;
+        double *pb = pub + (size_t)(i & 1) * pubsz;
+        // publish p (own rows >= i + 1), p'v, and row i + 1 from its owner
+        const int k0 = (i + 1 - g + G - 1) / G;   // first own row >= i + 1
+        for (int k = k0 + t; k < R; k += NT) coop_store(pb + (g + k * G - i - 1), pown[k]);
+        if (g == (i + 1) % G) {
+            const double *row = Ar + (size_t)((i + 1) / G) * ldr + i + 1;
+            for (int c = t; c < m; c += NT) coop_store(pb + n + c, row[c]);
+        }
+        if (!coop_barrier(sy, (unsigned)(i + 1), tick_limit, &flag)) return;
+        if (g == 0) {
+            for (int j = 1 + t; j < m; j += NT) A[(long)i * lda + i + 1 + j] = v[j];
+            if (t == 0) {
+                A[(long)i * lda + i] = dcur;
+                A[(long)i * lda + i + 1] = beta;
+                D[i] = dcur;
+                E[i] = beta;
+                tau[i] = ti;
+            }
+        }
+        // p, row i + 1 and the shares of p'v, loaded in one batch; every
+        // wave sums the shares in the same fixed order
+        constexpr int QM = (kCoopMaxN + NT - 1) / NT;
+        double pj[QM], rj[QM];
+#pragma unroll
+        for (int q = 0; q < QM; q++) {
+            const int j = t + q * NT;
+            pj[q] = j < m ? coop_load(pb + j) : 0.0;
+            rj[q] = j < m ? coop_load(pb + n + j) : 0.0;
+        }
+        const double p0 = coop_load(pb), r0 = coop_load(pb + n);
+        double pv = ln < G ? coop_load(pb + 2 * n + ln) : 0.0;
+        if (ln + 64 < G) pv += coop_load(pb + 2 * n + 64 + ln);
+        for (int off = 32; off > 0; off >>= 1) pv += __shfl_xor(pv, off, 64);
+        const double a2 = -0.5 * ti * (ti * pv);
+        // w = tau p + a2 v; the next column x = row(i+1) - v(0) w - w(0) v
+        // below its diagonal (v(0) = 1), with the squares of x(1:) summed
+        const double w0 = ti * p0 + a2 * 1.0;
+        dcur = r0 - (w0 + w0);
+        double ss = 0.0;
+#pragma unroll
+        for (int q = 0; q < QM; q++) {
+            const int j = t + q * NT;
+            if (j < m) {
+                const double wj = ti * pj[q] + a2 * v[j];
+                w[j] = wj;
+                if (j >= 1) {
+                    const double xv = rj[q] - (wj + w0 * v[j]);
+                    x[j - 1] = xv;
+                    if (j >= 2) ss += xv * xv;
+                }
+            }
+        }
+        if (m == 1) break;
+        for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
+        if (ln == 0) red[wv] = ss;
+        __syncthreads();
+        // Householder vector of x (dlarfg), the same bits in every workgroup
+        double xn2 = 0.0;
+        for (int q = 0; q < NW; q++) xn2 += red[q];
+        const double alpha = x[0];
+        double ti1 = 0.0, beta1 = alpha, scal = 1.0;
+        if (xn2 != 0.0) {
+            beta1 = -copysign(sqrt(alpha * alpha + xn2), alpha);
+            ti1 = (beta1 - alpha) / beta1;
+            scal = 1.0 / (alpha - beta1);
+        }
+        for (int j = t; j < m - 1; j += NT) vn[j] = j == 0 ? 1.0 : (ti1 != 0.0 ? x[j] * scal : x[j]);
+        __syncthreads();
+        // own rows r >= i + 2: rank-2 update of columns i+2..n-1 and the next
+        // step's p = A22 vn with this workgroup's share of p'vn; each wave
+        // takes four rows at a time (the v, w, vn loads shared, the four
+        // lane sums reduced together)
+        {
+            const int k1 = (i + 2 - g + G - 1) / G;
+            double dsum = 0.0;
+            for (int kb = k1 + wv; kb < R; kb += 4 * NW) {
+                double acc[4] = {0.0, 0.0, 0.0, 0.0}, vj[4], wj[4];
+                double *row[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int k = min(kb + u * NW, R - 1);
+                    const int j = g + k * G - i - 1;
+                    vj[u] = kb + u * NW < R ? v[j] : 0.0;
+                    wj[u] = kb + u * NW < R ? w[j] : 0.0;
+                    row[u] = Ar + (size_t)k * ldr + i + 1;
+                }
+                const int nu = min(4, (R - kb + NW - 1) / NW);
+                // two columns per lane and pass, every load issued before the
+                // stores (the LDS rows and vectors may alias for the compiler)
+                for (int c0 = 1 + ln; c0 < m; c0 += 128) {
+                    const int c1 = c0 + 64;
+                    const bool h1 = c1 < m;
+                    const double wc0 = w[c0], vc0 = v[c0], vnc0 = vn[c0 - 1];
+                    const double wc1 = h1 ? w[c1] : 0.0, vc1 = h1 ? v[c1] : 0.0;
+                    const double vnc1 = h1 ? vn[c1 - 1] : 0.0;
+                    double a0[4], a1[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        a0[u] = u < nu ? row[u][c0] : 0.0;
+                        a1[u] = u < nu && h1 ? row[u][c1] : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        a0[u] -= vj[u] * wc0 + wj[u] * vc0;
+                        a1[u] -= vj[u] * wc1 + wj[u] * vc1;
+                        acc[u] += a0[u] * vnc0;
+                        acc[u] += a1[u] * vnc1;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; u++)
+                        if (u < nu) {
+                            row[u][c0] = a0[u];
+                            if (h1) row[u][c1] = a1[u];
+                        }
+                }
+                for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+                    for (int u = 0; u < 4; u++) acc[u] += __shfl_xor(acc[u], off, 64);
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (u < nu) {
+                        const int k = kb + u * NW;
+                        if (ln == 0) pown[k] = acc[u];
+                        dsum += acc[u] * vn[g + k * G - i - 2];
+                    }
+            }
+            if (ln == 0) wdot[wv] = dsum;
+        }
+        __syncthreads();
+        if (t == 0) {
+            double sd = 0.0;
+            for (int q = 0; q < NW; q++) sd += wdot[q];
+            coop_store(pub + (size_t)((i + 1) & 1) * pubsz + 2 * n + g, sd);
+        }
+        double *tmp = v; v = vn; vn = tmp;
+        ti = ti1;
+        beta = beta1;
+    }
+    if (g == 0 && t == 0) {
+        A[(long)(n - 1) * lda + n - 1] = dcur;
+        D[n - 1] = dcur;
+        E[n - 1] = 0.0;
+        tau[n - 1] = 0.0;
+    }
+}
+
+// Workgroup count and dynamic LDS of sytrd_lower_coop for n (0 groups when
+// it does not apply): at least the requested count, more where the rows
+// would not fit in 150 KB of LDS per workgroup, at most one group per row.
+static int coop_plan(const sgpu_context *ctx, int n, size_t *shm)
+{
+    if (ctx->coop_tridiag <= 0 || n < 2 || n > kCoopMaxN) return 0;
+    const size_t lds_max = 150 * 1024;
+    const size_t fixed = (4 * (size_t)n + 2 * (ctx->coop_threads / 64)) * sizeof(double);
+    const size_t row = (size_t)(n + 2) * sizeof(double);   // the row and its p
+    if (fixed + row > lds_max) return 0;
+    const int rmax = (int)((lds_max - fixed) / row);
+    int G = std::max(ctx->coop_tridiag, (n + rmax - 1) / rmax);
+    G = std::min(G, n);
+    if (G > kCoopMaxG) return 0;
+    const int R = (n + G - 1) / G;
+    *shm = fixed + (size_t)R * row;
+    return G;
+}
+
+template <int NT>
+static int launch_sytrd_coop(sgpu_context *ctx, int n, int G, size_t shm, double *dA, double *dD,
+                             double *dE, double *dtau, double *pub, CoopSync *sy)
+{
+    int wc_khz = 0;
+    SGPU_HIP(hipDeviceGetAttribute(&wc_khz, hipDeviceAttributeWallClockRate, ctx->device));
+    // a barrier that waits 0.5 s is taken as a failed launch
+    long tick_limit = (long)std::max(wc_khz, 1000) * 500;
+    SGPU_HIP(hipFuncSetAttribute((const void *)sytrd_lower_coop<NT>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    SGPU_HIP(hipMemsetAsync(sy, 0, sizeof(CoopSync), ctx->stream));
+    int lda = n;
+    void *args[] = {&dA, &lda, &n, &dD, &dE, &dtau, &pub, &sy, &tick_limit};
+    SGPU_HIP(hipLaunchCooperativeKernel((const void *)sytrd_lower_coop<NT>, dim3(G), dim3(NT), args,
+                                        (unsigned)shm, ctx->stream));
+    return SGPU_OK;
+}
+
 int eigensolve_device(sgpu_context *ctx, int n, double *dA, double *dB, double *lambda)
 {
     void *buf = nullptr;
-    // D, E, tau, the tridiagonal eigenvectors and info in the shared scratch
-    // (free again once lambda is on the host)
-    SGPU_CHECK(sgpu_scratch(ctx, (3 * (size_t)n + (size_t)n * n) * sizeof(double) + 64, &buf));
+    // D, E, tau, the tridiagonal eigenvectors, the cooperative reduction's
+    // exchange buffers, its barrier and info in the shared scratch (free again
+    // once lambda is on the host)
+    const size_t npub = 2 * (2 * (size_t)n + kCoopMaxG);
+    SGPU_CHECK(sgpu_scratch(ctx, (3 * (size_t)n + (size_t)n * n + npub) * sizeof(double) +
+                                     sizeof(CoopSync) + 64,
+                            &buf));
     double *dD = (double *)buf, *dE = dD + n;
-    rocblas_int *dinfo = (rocblas_int *)(dE + 2 * n + (size_t)n * n);
-    sgpu_tmark t;
+    double *pub = dE + 2 * n + (size_t)n * n;
+    CoopSync *sy = (CoopSync *)(pub + npub);
+    rocblas_int *dinfo = (rocblas_int *)(sy + 1);
+    sgpu_tmark t, tp;
     SGPU_CHECK(sgpu_tbegin(ctx, &t));
+    SGPU_CHECK(sgpu_tbegin(ctx, &tp));
     rocblas_int info = 0;
     SGPU_BLAS(rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host));
+    sgpu_tmark tq;
+    SGPU_CHECK(sgpu_tbegin(ctx, &tq));
     SGPU_BLAS(rocsolver_dpotrf(ctx->blas, rocblas_fill_upper, n, dB, n, dinfo));
+    SGPU_CHECK(sgpu_tend(ctx, &tq, "eig_potrf"));
     SGPU_HIP(hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, ctx->stream));
     SGPU_HIP(hipStreamSynchronize(ctx->stream));
     if (info != 0) {
@@ -885,24 +1246,52 @@ int eigensolve_device(sgpu_context *ctx, int n, double *dA, double *dB, double *
     SGPU_BLAS(rocblas_dtrsm(ctx->blas, rocblas_side_right, rocblas_fill_upper,
                             rocblas_operation_none, rocblas_diagonal_non_unit, n, n, &one, dB, n,
                             dA, n));
-    if (n <= kTridMaxN && ctx->lds_tridiag) {
-        // tridiagonal reduction in one workgroup, then the tridiagonal
-        // eigenvectors (dstedc) and the back-transformation Q Z (dormtr):
-        // the three steps of dsyevd with its dsytrd replaced
+    SGPU_CHECK(sgpu_tend(ctx, &tp, "eig_cholesky"));
+    size_t coop_shm = 0;
+    const bool lds = n <= kTridMaxN && ctx->lds_tridiag;
+    const int G = lds ? 0 : coop_plan(ctx, n, &coop_shm);
+    if (lds || G > 0) {
+        // tridiagonal reduction (one workgroup, or one cooperative launch),
+        // then the tridiagonal eigenvectors (dstedc) and the
+        // back-transformation Q Z (dormtr): the three steps of dsyevd with its
+        // dsytrd replaced
         double *dtau = dE + n, *dC = dtau + n;
-        const size_t shm = ((size_t)n * (n + 1) / 2 + 2 * (size_t)n + kTridThreads) * sizeof(double);
-        SGPU_HIP(hipFuncSetAttribute((const void *)sytd2_lower_lds,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-        sytd2_lower_lds<<<1, kTridThreads, shm, ctx->stream>>>(dA, n, n, dD, dE, dtau);
-        SGPU_HIP(hipGetLastError());
+        SGPU_CHECK(sgpu_tbegin(ctx, &tp));
+        if (lds) {
+            const size_t shm = ((size_t)n * (n + 1) / 2 + 2 * (size_t)n + kTridThreads) * sizeof(double);
+            SGPU_HIP(hipFuncSetAttribute((const void *)sytd2_lower_lds,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+            sytd2_lower_lds<<<1, kTridThreads, shm, ctx->stream>>>(dA, n, n, dD, dE, dtau);
+            SGPU_HIP(hipGetLastError());
+        } else {
+            if (ctx->coop_threads == 512)
+                SGPU_CHECK(launch_sytrd_coop<512>(ctx, n, G, coop_shm, dA, dD, dE, dtau, pub, sy));
+            else
+                SGPU_CHECK(launch_sytrd_coop<256>(ctx, n, G, coop_shm, dA, dD, dE, dtau, pub, sy));
+            CoopSync hs{};
+            SGPU_HIP(hipMemcpyAsync(&hs, sy, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
+            SGPU_HIP(hipStreamSynchronize(ctx->stream));
+            if (hs.abort) {
+                sgpu_set_error("eigensolve: tridiagonal reduction's grid barrier timed out "
+                               "(%d workgroups)", G);
+                return SGPU_ERR_SOLVER;
+            }
+        }
+        SGPU_CHECK(sgpu_tend(ctx, &tp, "eig_tridiag"));
+        SGPU_CHECK(sgpu_tbegin(ctx, &tp));
         SGPU_BLAS(rocsolver_dstedc(ctx->blas, rocblas_evect_tridiagonal, n, dD, dE, dC, n, dinfo));
+        SGPU_CHECK(sgpu_tend(ctx, &tp, "eig_stedc"));
+        SGPU_CHECK(sgpu_tbegin(ctx, &tp));
+        SGPU_CHECK(sgpu_tbegin(ctx, &tq));
         SGPU_BLAS(rocsolver_dormtr(ctx->blas, rocblas_side_left, rocblas_fill_lower,
                                    rocblas_operation_none, n, n, dA, n, dtau, dC, n));
+        SGPU_CHECK(sgpu_tend(ctx, &tq, "eig_ormtr"));
         SGPU_BLAS(rocblas_dtrsm(ctx->blas, rocblas_side_left, rocblas_fill_upper,
                                 rocblas_operation_none, rocblas_diagonal_non_unit, n, n, &one, dB,
                                 n, dC, n));
         SGPU_HIP(hipMemcpyAsync(dA, dC, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice,
                                 ctx->stream));
+        SGPU_CHECK(sgpu_tend(ctx, &tp, "eig_back"));
     } else {
         SGPU_BLAS(rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, n, dA, n,
                                    dD, dE, dinfo));
@@ -915,7 +1304,7 @@ int eigensolve_device(sgpu_context *ctx, int n, double *dA, double *dB, double *
     SGPU_HIP(hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, ctx->stream));
     SGPU_HIP(hipStreamSynchronize(ctx->stream));
     if (info != 0) {
-        sgpu_set_error("eigensolve: dsyevd did not converge (info = %d)", (int)info);
+        sgpu_set_error("eigensolve: tridiagonal eigensolve did not converge (info = %d)", (int)info);
         return SGPU_ERR_SOLVER;
     }
     return SGPU_OK;

@@ -141,6 +141,11 @@ struct sgpu_context {
     // workgroup's LDS instead of rocsolver's dsytrd (SPARC_AMD_LDS_TRIDIAG=0
     // keeps dsyevd everywhere)
     bool lds_tridiag = true;
+    // larger eigensolves reduce to tridiagonal form in one cooperative launch
+    // of this many workgroups, each holding a share of the rows in LDS
+    // (SPARC_AMD_COOP_TRIDIAG=<groups>; 0 keeps dsyevd)
+    int coop_tridiag = 32;
+    int coop_threads = 256;
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;
 

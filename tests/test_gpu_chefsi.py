@@ -294,3 +294,61 @@ def test_filter_nonlocal_chain(gpu, oracle_mod, monkeypatch, mode, overlap, dtyp
             out[chain] = by.download()
     assert np.array_equal(out["1"], out["0"])
     assert rel_err(out["1"], Yo) < (1e-11 if dtype == 0 else 1e-4)
+
+
+@pytest.mark.parametrize("n,groups,threads", [(181, 32, 256), (300, 7, 256), (480, 32, 256),
+                                              (480, 64, 512), (1000, 16, 256)])
+def test_generalized_eigensolve_cooperative_tridiagonal(gpu, monkeypatch, n, groups, threads):
+    """Above 180 states the reduction to tridiagonal form runs in one
+    cooperative launch, the rows spread over the workgroups' LDS (dsytd2 'L'
+    with one grid barrier per column), then dstedc + dormtr: eigenvalues
+    against LAPACK dsygvd and rocsolver's dsyevd path
+    (SPARC_AMD_COOP_TRIDIAG=0) to 1e-10, Mp-orthonormal eigenvectors that
+    diagonalize Hp, on the filtered-block scale spread; group counts that do
+    not divide n, and n = 1000 (more groups than asked, for the LDS)."""
+    from sparc_amd.workloads import Grid
+    rng = np.random.default_rng(90 + n + groups)
+    d = 10.0 ** rng.uniform(-4, 4, n)
+    lam0 = np.sort(rng.uniform(-0.3, 0.5, n))
+    E = 1e-3 * rng.standard_normal((n, n))
+    Mp = np.sqrt(np.outer(d, d)) * (np.eye(n) + (E + E.T) / 2)
+    F = 1e-3 * rng.standard_normal((n, n))
+    Hp = np.sqrt(np.outer(d, d)) * (np.diag(lam0) + (F + F.T) / 2)
+    g = Grid(8, 8, 8, 0.5, 0.5, 0.5)
+    out = {}
+    monkeypatch.setenv("SPARC_AMD_COOP_THREADS", str(threads))
+    for flag in (str(groups), "0"):
+        monkeypatch.setenv("SPARC_AMD_COOP_TRIDIAG", flag)
+        with gpu.Context(g) as ctx:
+            out[flag] = ctx.DP_Solve_Generalized_EigenProblem(Hp, Mp)
+    lam, Q = out[str(groups)]
+    lam_ref = sla.eigh(Hp, Mp, eigvals_only=True, driver="gvd")
+    assert np.max(np.abs(lam - lam_ref)) < 1e-10
+    assert np.max(np.abs(lam - out["0"][0])) < 1e-10
+    assert np.max(np.abs(Q.T @ Mp @ Q - np.eye(n))) < 1e-9
+    assert np.max(np.abs(Q.T @ Hp @ Q - np.diag(lam))) < 1e-9
+
+
+def test_generalized_eigensolve_cooperative_degenerate(gpu):
+    """Exactly degenerate eigenvalues (a cubic crystal's threefold states) and
+    a zero column below the diagonal (the Householder step with tau = 0):
+    n = 240 built as V diag(lam) V' with triplets, and a block-diagonal
+    matrix whose first column is e_0."""
+    from sparc_amd.workloads import Grid
+    rng = np.random.default_rng(44)
+    n = 240
+    lam0 = np.repeat(np.sort(rng.uniform(-0.4, 0.6, n // 3)), 3)
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    Hp = (V * lam0) @ V.T
+    Hp = 0.5 * (Hp + Hp.T)
+    Hb = np.zeros((n, n))
+    Hb[0, 0] = 0.7
+    B = rng.standard_normal((n - 1, n - 1))
+    Hb[1:, 1:] = B + B.T
+    Mp = np.eye(n)
+    with gpu.Context(Grid(8, 8, 8, 0.5, 0.5, 0.5)) as ctx:
+        for H in (Hp, Hb):
+            lam, Q = ctx.DP_Solve_Generalized_EigenProblem(H, Mp)
+            assert np.max(np.abs(lam - np.linalg.eigvalsh(H))) < 1e-10
+            assert np.max(np.abs(Q.T @ Q - np.eye(n))) < 1e-10
+            assert np.max(np.abs(Q.T @ H @ Q - np.diag(lam))) < 1e-9
