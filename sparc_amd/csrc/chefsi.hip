@@ -724,13 +724,15 @@ int rotate_raw(sgpu_context *ctx, int dtype, int ns, const void *Y, long ldy, co
 }
 
 // dsygvd on the device (FP64), in place: dA (Hp, n x n) receives Q, dB (Mp)
-// its Cholesky factor; lambda (host) the eigenvalues in ascending order.
-// Hp Q = Mp Q Lambda with Q' Mp Q = I (LAPACKE_dsygvd itype 1 in the
-// reference, src/eigenSolver.c:1262-1375), solved as Mp = U'U (dpotrf),
-// C = U^-T Hp U^-1 (two dtrsm), C Z = Z Lambda (dsyevd), Q = U^-1 Z (dtrsm).
-// Same reduction as dsygvd, but rocSOLVER's dsygst took 16 ms of dsygvd's
-// 29 ms at n = 480 on MI355X where the two triangular solves take well under
-// one.
+// is overwritten (by L^-1, or by U with SPARC_AMD_CHOL=rocsolver); lambda
+// (host) the eigenvalues in ascending order.  Hp Q = Mp Q Lambda with
+// Q' Mp Q = I (LAPACKE_dsygvd itype 1 in the reference,
+// src/eigenSolver.c:1262-1375), solved as Mp = L L' (chol_panel), L^-1
+// (dtrtri), C = L^-1 Hp L^-T (two dgemm), C Z = Z Lambda (tridiagonal
+// reduction, dstedc, dormtr) and Q = L^-T Z (dgemm).  Same reduction as
+// dsygvd; at n = 480 on MI355X rocSOLVER's dsygst takes 16 ms of dsygvd's
+// 29 ms, its dpotrf 1.06 ms and each dtrsm 0.16-0.18 ms, against 0.02 ms for
+// a dgemm of that size.
 // Householder reduction of a symmetric matrix to tridiagonal form inside one
 // workgroup, the matrix held in LDS: LAPACK dsytd2 with UPLO = 'L' (the
 // storage rocsolver_dormtr reads back): for i = 0..n-2, H(i) = I - tau v v'
@@ -1172,6 +1174,189 @@ __global__ __launch_bounds__(NT) void sytrd_lower_coop(double *__restrict__ A, i
     }
 }
 
+// Cholesky factor of the overlap matrix, Mp = L L' (lower), left-looking in
+// panels of 32 columns, one launch per panel: LAPACK dpotrf's factor (the
+// reference's dsygvd, src/eigenSolver.c:1318, starts with it), where
+// rocSOLVER's dpotrf spends ~1 ms in short launches at 480 states.  Block 0
+// of a launch owns the diagonal block, block b >= 1 the 32 rows
+// kb + nb + 32 (b - 1) ... below it.  Every block forms the diagonal block
+// D = A11 - L10 L10' (the same bits in every block) and, for a row block,
+// P = A21 - L20 L10': the columns of L10 / L20 are staged four chunks of 32
+// at a time (one chunk per wave, the four partial sums added in a fixed
+// order).  One wave factors D in registers (rows in lanes, the pivot column
+// broadcast with readlane); the row blocks then solve L21 L11' = P row by
+// row.  Block 0 also zeroes the strict upper triangle of the panel's columns
+// (the factor is then used as a full matrix).  info gets the 1-based column
+// of the first pivot that is not positive, and a launch after a failed one
+// returns at once.
+constexpr int kCholNB = 32;
+constexpr int kCholThreads = 256;
+constexpr int kCholLd = kCholNB + 1;
+constexpr size_t kCholShm = (4 * 2 + 3) * (size_t)kCholNB * kCholLd * sizeof(double) +
+                            kCholNB * sizeof(double);
+
+__device__ __forceinline__ double readlane_f64(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__global__ __launch_bounds__(kCholThreads) void chol_panel(double *__restrict__ B, int n, int kb,
+                                                           int *__restrict__ info)
+{
+    extern __shared__ double csm[];
+    double(*St)[kCholLd] = (double(*)[kCholLd])csm;             // [4 chunks][Ld; Lr][32] rows
+    double(*Dm)[kCholLd] = St + 8 * kCholNB;
+    double(*Pm)[kCholLd] = Dm + kCholNB;
+    double(*L11)[kCholLd] = Pm + kCholNB;
+    double *rdiag = (double *)(L11 + kCholNB);
+    __shared__ int fail;
+    if (*info != 0) return;
+    const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
+    const int nb = min(kCholNB, n - kb);
+    const int b = blockIdx.x;
+    const int r0 = kb + nb + kCholNB * (b - 1);
+    const int nr = b == 0 ? 0 : min(kCholNB, n - r0);
+    // lane tile: rows ri + 8 a, columns cj + 8 c (a, c < 4)
+    const int ri = ln >> 3, cj = ln & 7;
+    double ad[4][4] = {}, ap[4][4] = {};
+    for (int s0 = 0; s0 < kb; s0 += 4 * kCholNB) {
+        const int nch = min(4, (kb - s0) / kCholNB);
+        for (int e = t; e < nch * kCholNB * kCholNB; e += kCholThreads) {
+            const int ii = e & 31, kk = (e >> 5) & 31, ch = e >> 10;
+            const double *col = B + (long)(s0 + ch * kCholNB + kk) * n;
+            St[(2 * ch) * kCholNB + ii][kk] = ii < nb ? col[kb + ii] : 0.0;
+            St[(2 * ch + 1) * kCholNB + ii][kk] = ii < nr ? col[r0 + ii] : 0.0;
+        }
+        __syncthreads();
+        if (wv < nch) {
+            const double(*Ld)[kCholLd] = St + (2 * wv) * kCholNB;
+            const double(*Lr)[kCholLd] = St + (2 * wv + 1) * kCholNB;
+#pragma unroll 4
+            for (int kk = 0; kk < kCholNB; kk++) {
+                double li[4], lr[4], lj[4];
+#pragma unroll
+                for (int a = 0; a < 4; a++) {
+                    li[a] = Ld[ri + 8 * a][kk];
+                    lr[a] = Lr[ri + 8 * a][kk];
+                    lj[a] = Ld[cj + 8 * a][kk];
+                }
+#pragma unroll
+                for (int a = 0; a < 4; a++)
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        ad[a][c] += li[a] * lj[c];
+                        ap[a][c] += lr[a] * lj[c];
+                    }
+            }
+        }
+        __syncthreads();
+    }
+    // partial sums of the four waves, added in wave order
+    if (kb > 0) {
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                St[(2 * wv) * kCholNB + ri + 8 * a][cj + 8 * c] = ad[a][c];
+                St[(2 * wv + 1) * kCholNB + ri + 8 * a][cj + 8 * c] = ap[a][c];
+            }
+        __syncthreads();
+    }
+    for (int e = t; e < kCholNB * kCholNB; e += kCholThreads) {
+        const int i = e >> 5, j = e & 31;
+        double sd = 0.0, sp = 0.0;
+        if (kb > 0)
+            for (int w = 0; w < 4; w++) {
+                sd += St[(2 * w) * kCholNB + i][j];
+                sp += St[(2 * w + 1) * kCholNB + i][j];
+            }
+        double d;
+        if (i < nb && j < nb)
+            d = j <= i ? B[(long)(kb + j) * n + kb + i] - sd : 0.0;
+        else
+            d = i == j ? 1.0 : 0.0;
+        Dm[i][j] = d;
+        Pm[i][j] = i < nr && j < nb ? B[(long)(kb + j) * n + r0 + i] - sp : 0.0;
+    }
+    __syncthreads();
+    // L11 by one wave: lane l < 32 holds row l of D; step j scales column j
+    // by its pivot and subtracts L(l, j) L(k, j) from row l (k > j)
+    if (wv == 0) {
+        const int l = ln & 31;
+        double row[kCholNB];
+#pragma unroll
+        for (int k = 0; k < kCholNB; k++) row[k] = Dm[l][k];
+        int bad = 0;
+#pragma unroll
+        for (int j = 0; j < kCholNB; j++) {
+            const double djj = readlane_f64(row[j], j);
+            if (!(djj > 0.0)) {
+                bad = j + 1;
+                break;
+            }
+            const double piv = sqrt(djj), rp = 1.0 / piv;
+            row[j] = l == j ? piv : row[j] * rp;
+#pragma unroll
+            for (int k = j + 1; k < kCholNB; k++) row[k] -= row[j] * readlane_f64(row[j], k);
+        }
+        if (ln < kCholNB) {
+#pragma unroll
+            for (int k = 0; k < kCholNB; k++) L11[l][k] = k <= l ? row[k] : 0.0;
+            rdiag[l] = 1.0 / row[l];
+        }
+        if (ln == 0) fail = bad;
+    }
+    __syncthreads();
+    if (fail) {
+        if (b == 0 && t == 0) *info = kb + fail;
+        return;
+    }
+    if (b == 0) {
+        for (int e = t; e < kCholNB * kCholNB; e += kCholThreads) {
+            const int ii = e & 31, jj = e >> 5;
+            if (ii < nb && jj < nb) B[(long)(kb + jj) * n + kb + ii] = L11[ii][jj];
+        }
+        for (int jj = 0; jj < nb; jj++)
+            for (int r = t; r < kb; r += kCholThreads) B[(long)(kb + jj) * n + r] = 0.0;
+        return;
+    }
+    // L21 = P L11^-T: one thread per row, forward substitution
+    if (t < nr) {
+        double x[kCholNB];
+#pragma unroll
+        for (int j = 0; j < kCholNB; j++) {
+            double s = Pm[t][j];
+#pragma unroll
+            for (int k = 0; k < j; k++) s -= x[k] * L11[j][k];
+            x[j] = s * rdiag[j];
+        }
+#pragma unroll
+        for (int j = 0; j < kCholNB; j++) Pm[t][j] = x[j];
+    }
+    __syncthreads();
+    for (int e = t; e < kCholNB * kCholNB; e += kCholThreads) {
+        const int ii = e & 31, jj = e >> 5;
+        if (ii < nr && jj < nb) B[(long)(kb + jj) * n + r0 + ii] = Pm[ii][jj];
+    }
+}
+
+static int cholesky_device(sgpu_context *ctx, int n, double *dB, rocblas_int *dinfo)
+{
+    SGPU_HIP(hipFuncSetAttribute((const void *)chol_panel,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)kCholShm));
+    SGPU_HIP(hipMemsetAsync(dinfo, 0, sizeof(rocblas_int), ctx->stream));
+    for (int kb = 0; kb < n; kb += kCholNB) {
+        const int nb = std::min(kCholNB, n - kb);
+        const int nblk = 1 + (n - kb - nb + kCholNB - 1) / kCholNB;
+        chol_panel<<<nblk, kCholThreads, kCholShm, ctx->stream>>>(dB, n, kb, (int *)dinfo);
+        SGPU_HIP(hipGetLastError());
+    }
+    return SGPU_OK;
+}
+
 // Workgroup count and dynamic LDS of sytrd_lower_coop for n (0 groups when
 // it does not apply): at least the requested count, more where the rows
 // would not fit in 150 KB of LDS per workgroup, at most one group per row.
@@ -1230,22 +1415,36 @@ int eigensolve_device(sgpu_context *ctx, int n, double *dA, double *dB, double *
     SGPU_BLAS(rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host));
     sgpu_tmark tq;
     SGPU_CHECK(sgpu_tbegin(ctx, &tq));
-    SGPU_BLAS(rocsolver_dpotrf(ctx->blas, rocblas_fill_upper, n, dB, n, dinfo));
+    if (ctx->own_chol)
+        SGPU_CHECK(cholesky_device(ctx, n, dB, dinfo));
+    else
+        SGPU_BLAS(rocsolver_dpotrf(ctx->blas, rocblas_fill_upper, n, dB, n, dinfo));
     SGPU_CHECK(sgpu_tend(ctx, &tq, "eig_potrf"));
     SGPU_HIP(hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, ctx->stream));
     SGPU_HIP(hipStreamSynchronize(ctx->stream));
     if (info != 0) {
-        sgpu_set_error("eigensolve: overlap matrix not positive definite (dpotrf info = %d)",
+        sgpu_set_error("eigensolve: overlap matrix not positive definite (Cholesky info = %d)",
                        (int)info);
         return SGPU_ERR_SOLVER;
     }
-    const double one = 1.0;
-    SGPU_BLAS(rocblas_dtrsm(ctx->blas, rocblas_side_left, rocblas_fill_upper,
-                            rocblas_operation_transpose, rocblas_diagonal_non_unit, n, n, &one, dB,
-                            n, dA, n));
-    SGPU_BLAS(rocblas_dtrsm(ctx->blas, rocblas_side_right, rocblas_fill_upper,
-                            rocblas_operation_none, rocblas_diagonal_non_unit, n, n, &one, dB, n,
-                            dA, n));
+    const double one = 1.0, zero = 0.0;
+    double *dW = dE + 2 * n;   // n x n work (the tridiagonal eigenvectors later)
+    if (ctx->own_chol) {
+        // dB <- L^-1 (lower, zero above), C = L^-1 Hp L^-T
+        SGPU_BLAS(rocsolver_dtrtri(ctx->blas, rocblas_fill_lower, rocblas_diagonal_non_unit, n, dB,
+                                   n, dinfo));
+        SGPU_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n,
+                                &one, dB, n, dA, n, &zero, dW, n));
+        SGPU_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_transpose, n,
+                                n, n, &one, dW, n, dB, n, &zero, dA, n));
+    } else {
+        SGPU_BLAS(rocblas_dtrsm(ctx->blas, rocblas_side_left, rocblas_fill_upper,
+                                rocblas_operation_transpose, rocblas_diagonal_non_unit, n, n, &one,
+                                dB, n, dA, n));
+        SGPU_BLAS(rocblas_dtrsm(ctx->blas, rocblas_side_right, rocblas_fill_upper,
+                                rocblas_operation_none, rocblas_diagonal_non_unit, n, n, &one, dB,
+                                n, dA, n));
+    }
     SGPU_CHECK(sgpu_tend(ctx, &tp, "eig_cholesky"));
     size_t coop_shm = 0;
     const bool lds = n <= kTridMaxN && ctx->lds_tridiag;
@@ -1286,18 +1485,30 @@ int eigensolve_device(sgpu_context *ctx, int n, double *dA, double *dB, double *
         SGPU_BLAS(rocsolver_dormtr(ctx->blas, rocblas_side_left, rocblas_fill_lower,
                                    rocblas_operation_none, n, n, dA, n, dtau, dC, n));
         SGPU_CHECK(sgpu_tend(ctx, &tq, "eig_ormtr"));
-        SGPU_BLAS(rocblas_dtrsm(ctx->blas, rocblas_side_left, rocblas_fill_upper,
-                                rocblas_operation_none, rocblas_diagonal_non_unit, n, n, &one, dB,
-                                n, dC, n));
-        SGPU_HIP(hipMemcpyAsync(dA, dC, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice,
-                                ctx->stream));
+        if (ctx->own_chol) {
+            SGPU_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none,
+                                    n, n, n, &one, dB, n, dC, n, &zero, dA, n));
+        } else {
+            SGPU_BLAS(rocblas_dtrsm(ctx->blas, rocblas_side_left, rocblas_fill_upper,
+                                    rocblas_operation_none, rocblas_diagonal_non_unit, n, n, &one,
+                                    dB, n, dC, n));
+            SGPU_HIP(hipMemcpyAsync(dA, dC, (size_t)n * n * sizeof(double),
+                                    hipMemcpyDeviceToDevice, ctx->stream));
+        }
         SGPU_CHECK(sgpu_tend(ctx, &tp, "eig_back"));
     } else {
         SGPU_BLAS(rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, n, dA, n,
                                    dD, dE, dinfo));
-        SGPU_BLAS(rocblas_dtrsm(ctx->blas, rocblas_side_left, rocblas_fill_upper,
-                                rocblas_operation_none, rocblas_diagonal_non_unit, n, n, &one, dB,
-                                n, dA, n));
+        if (ctx->own_chol) {
+            SGPU_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none,
+                                    n, n, n, &one, dB, n, dA, n, &zero, dW, n));
+            SGPU_HIP(hipMemcpyAsync(dA, dW, (size_t)n * n * sizeof(double),
+                                    hipMemcpyDeviceToDevice, ctx->stream));
+        } else {
+            SGPU_BLAS(rocblas_dtrsm(ctx->blas, rocblas_side_left, rocblas_fill_upper,
+                                    rocblas_operation_none, rocblas_diagonal_non_unit, n, n, &one,
+                                    dB, n, dA, n));
+        }
     }
     SGPU_CHECK(sgpu_tend(ctx, &t, "eigensolve"));
     SGPU_HIP(hipMemcpyAsync(lambda, dD, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));

@@ -146,6 +146,10 @@ struct sgpu_context {
     // (SPARC_AMD_COOP_TRIDIAG=<groups>; 0 keeps dsyevd)
     int coop_tridiag = 32;
     int coop_threads = 256;
+    // the eigensolve's overlap factor by the blocked Cholesky kernels below,
+    // with C = L^-1 Hp L^-T and Q = L^-T Z as products with L^-1
+    // (SPARC_AMD_CHOL=rocsolver keeps dpotrf and the triangular solves)
+    bool own_chol = true;
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;
 

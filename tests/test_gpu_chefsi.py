@@ -352,3 +352,41 @@ def test_generalized_eigensolve_cooperative_degenerate(gpu):
             assert np.max(np.abs(lam - np.linalg.eigvalsh(H))) < 1e-10
             assert np.max(np.abs(Q.T @ Q - np.eye(n))) < 1e-10
             assert np.max(np.abs(Q.T @ H @ Q - np.diag(lam))) < 1e-9
+
+
+@pytest.mark.parametrize("n", [24, 181, 480])
+def test_generalized_eigensolve_blocked_cholesky(gpu, monkeypatch, n):
+    """The overlap factor from the blocked Cholesky kernels (panels of 32
+    columns; n = 181 leaves a partial panel) with C = L^-1 Hp L^-T and
+    Q = L^-T Z as products, against rocSOLVER's dpotrf + triangular solves
+    (SPARC_AMD_CHOL=rocsolver) and LAPACK dsygvd: eigenvalues to 1e-10,
+    Mp-orthonormal eigenvectors; a matrix that is not positive definite in
+    its last panel raises, with the column of the failing pivot."""
+    from sparc_amd.workloads import Grid
+    rng = np.random.default_rng(120 + n)
+    d = 10.0 ** rng.uniform(-4, 4, n)
+    lam0 = np.sort(rng.uniform(-0.3, 0.5, n))
+    E = 1e-3 * rng.standard_normal((n, n))
+    Mp = np.sqrt(np.outer(d, d)) * (np.eye(n) + (E + E.T) / 2)
+    F = 1e-3 * rng.standard_normal((n, n))
+    Hp = np.sqrt(np.outer(d, d)) * (np.diag(lam0) + (F + F.T) / 2)
+    g = Grid(8, 8, 8, 0.5, 0.5, 0.5)
+    out = {}
+    for flag in ("own", "rocsolver"):
+        monkeypatch.setenv("SPARC_AMD_CHOL", flag)
+        with gpu.Context(g) as ctx:
+            out[flag] = ctx.DP_Solve_Generalized_EigenProblem(Hp, Mp)
+    lam, Q = out["own"]
+    lam_ref = sla.eigh(Hp, Mp, eigvals_only=True, driver="gvd")
+    assert np.max(np.abs(lam - lam_ref)) < 1e-10
+    assert np.max(np.abs(lam - out["rocsolver"][0])) < 1e-10
+    assert np.max(np.abs(Q.T @ Mp @ Q - np.eye(n))) < 1e-9
+    assert np.max(np.abs(Q.T @ Hp @ Q - np.diag(lam))) < 1e-9
+    bad = np.eye(n)
+    bad[n - 1, n - 1] = -1.0
+    monkeypatch.setenv("SPARC_AMD_CHOL", "own")
+    with gpu.Context(g) as ctx:
+        with pytest.raises(gpu.SparcGPUError, match=f"info = {n}"):
+            ctx.DP_Solve_Generalized_EigenProblem(np.eye(n), bad)
+        lam2, _ = ctx.DP_Solve_Generalized_EigenProblem(Hp, Mp)   # usable afterwards
+    assert np.max(np.abs(lam2 - lam_ref)) < 1e-10
