@@ -577,11 +577,41 @@ int gram_mode()
 // symmetric: C is symmetric (Y'Y, or Y'HY with H symmetric); the 128 x 128
 // kernel then forms the tiles on and above the diagonal only and mirrors
 // them, which is what the reference's dsygvd reads of Hp and Mp ('U').
+// K-slice of gram3_bf16 (one 512-thread block per CU): the number of
+// (tile, slice) blocks is chosen so that they fill whole rounds of the CUs,
+// minimising rounds x slice rows with slices of at most 8192 rows (multiples
+// of the 32-point chunk).  216-atom cell (474,552 rows, 10 tiles): 102
+// slices of 4,672 rows in four full rounds instead of 58 of 8,192 in three
+// rounds with the last one 27 % full (Mp + Hp 4.19 -> 3.30 ms on MI355X).
+static long gram3_kslice(const sgpu_context *ctx, long nblk, long nd)
+{
+    const long cap = 8192, cus = std::max(1, ctx->n_cu);
+    const long ns0 = std::max(1L, (nd + cap - 1) / cap);
+    long best = std::min(cap, std::max(nd, 1L)), best_cost = -1;
+    for (long ns = ns0; ns <= 2 * ns0 + 8; ns++) {
+        long ks = (nd + ns - 1) / ns;
+        ks = (ks + kG3K - 1) / kG3K * kG3K;
+        if (ks > cap) continue;
+        const long nsl = (nd + ks - 1) / ks;
+        const long cost = (nblk * nsl + cus - 1) / cus * ks;
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = ks;
+        }
+    }
+    return best;
+}
+
 template <typename T>
 int gram_tn_typed(sgpu_context *ctx, int n, long nd, const T *A, long lda, const T *B, long ldb,
                   double *C, bool symmetric)
 {
-    const long kslice = gram_kslice(n, nd, sizeof(T) == 4 ? SGPU_F32 : SGPU_F64);
+    long kslice = gram_kslice(n, nd, sizeof(T) == 4 ? SGPU_F32 : SGPU_F64);
+    const bool g3 = sizeof(T) == 4 && n >= 256 && gram_mode() == 0;
+    const int tiles3 = (n + kG3T - 1) / kG3T;
+    const bool sym3 = symmetric || (A == B && lda == ldb);
+    const unsigned nblk3 = sym3 ? (unsigned)(tiles3 * (tiles3 + 1) / 2) : (unsigned)(tiles3 * tiles3);
+    if (g3) kslice = gram3_kslice(ctx, nblk3, nd);
     const int nslice = (int)((nd + kslice - 1) / kslice);
     const long nn = (long)n * n;
     void *buf = nullptr;
@@ -591,13 +621,11 @@ int gram_tn_typed(sgpu_context *ctx, int n, long nd, const T *A, long lda, const
     // 128 x 128 one (512k x 480: 4.8 ms for Mp + Hp, rocBLAS sgemm 4.7 ms;
     // the 64 x 64 kernel took 15 ms there, bound by splitting every staged
     // value once per 64 output columns)
-    if (sizeof(T) == 4 && n >= 256 && gram_mode() == 0) {
+    if (g3) {
         // 128 x 128 tiles of the gfx950 BF16-piece kernel; Mp = Y'Y computes
         // the upper triangle of tiles only
-        const int tiles = (n + kG3T - 1) / kG3T;
-        const bool sym = symmetric || (A == B && lda == ldb);
-        const unsigned nblk = sym ? (unsigned)(tiles * (tiles + 1) / 2) : (unsigned)(tiles * tiles);
-        if (sym)
+        const unsigned nblk = nblk3;
+        if (sym3)
             gram3_bf16<true><<<dim3(nblk, nslice), 512, 0, ctx->stream>>>(
                 (const float *)A, lda, (const float *)B, ldb, n, nd, kslice, (float *)part);
         else

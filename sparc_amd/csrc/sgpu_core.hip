@@ -77,6 +77,12 @@ extern "C" int sgpu_create(sgpu_context **out, int device, int nx, int ny, int n
     if (const char *t = getenv("SPARC_AMD_COOP_TRIDIAG")) ctx->coop_tridiag = std::max(0, atoi(t));
     if (const char *t = getenv("SPARC_AMD_COOP_THREADS")) ctx->coop_threads = atoi(t) == 512 ? 512 : 256;
     if (const char *t = getenv("SPARC_AMD_CHOL")) ctx->own_chol = strcmp(t, "rocsolver") != 0;
+    {
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+            ncu > 0)
+            ctx->n_cu = ncu;
+    }
     // second-derivative weights: w_p = 2 (-1)^{p+1} ratio(fdn,p) / p^2,
     // w_0 = -2 sum 1/p^2, divided by h^2 per axis
     double w[SGPU_MAX_FDN + 1];

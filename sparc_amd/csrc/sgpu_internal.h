@@ -150,6 +150,7 @@ struct sgpu_context {
     // with C = L^-1 Hp L^-T and Q = L^-T Z as products with L^-1
     // (SPARC_AMD_CHOL=rocsolver keeps dpotrf and the triangular solves)
     bool own_chol = true;
+    int n_cu = 256;   // compute units of the device (set by sgpu_create)
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;
 
