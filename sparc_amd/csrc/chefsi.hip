@@ -266,18 +266,22 @@ __global__ __launch_bounds__(256) void gram_bf16x3(const float *__restrict__ A, 
 // blocks: every value is split into three BF16 pieces (as in gram_bf16x3) and
 // the six products with i + j <= 2 are summed on v_mfma_f32_16x16x32_bf16
 // (16 x 16 output tile, K = 32 per instruction, FP32 accumulation).  One block
-// takes a 64-point strip of Y and produces all n output columns, so Y is read
+// takes a PTS-point strip of Y and produces all n output columns, so Y is read
 // and split once: the strip's 32-column K-chunks are staged in LDS as
 // [piece][point][k] BF16 (the A operands, 16-byte reads), while Q comes as
 // three BF16 planes prepared once per rotation (split_q_bf16x3), column-major
-// with K padded to a multiple of 32 (the B operands, 16-byte loads from L2).
-// The eight waves share the strip and take every eighth 16-column output tile.
+// with K padded to a multiple of 32 (the B operands, 16-byte loads from L2,
+// issued with the chunk's Y loads).  The eight waves share the strip and take
+// every eighth 16-column output tile.  Every block reads all of Q, so the
+// strip sets the L2 traffic for Q: 128-point strips (253 VGPRs, still two
+// waves per SIMD) took 1.45 ms at 474,552 x 480 on MI355X against 1.82 ms
+// for 64-point strips.
 // Operand layout of the 16x16x32 instruction: A(i, k) in lane i + 16 (k / 8),
 // element k % 8; B(k, n) in lane n + 16 (k / 8), element k % 8; D(i, n) in
 // lane n + 16 (i / 4), register i % 4, so each lane's four results are four
 // consecutive points of one column (one 16-byte store).
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
-constexpr int kRotPts = 64;        // points per block
+constexpr int kRotPts = 128;       // points per block
 constexpr int kRotK = 32;          // K per chunk
 constexpr int kRotLd = kRotK + 8;  // LDS row (bf16): 80 bytes
 constexpr int kRotWaves = 8;       // waves per block
@@ -308,40 +312,51 @@ __global__ void split_q_bf16x3(const double *__restrict__ q, int n, int nk,
     qp[2 * tot + t] = p2;
 }
 
+template <int PTS>
 __global__ __launch_bounds__(64 * kRotWaves) void rot_bf16x3(const float *__restrict__ Y, long ldy,
                                                   const unsigned short *__restrict__ qp, int n,
                                                   int nk, long nd, float *__restrict__ out,
                                                   long ldo)
 {
-    __shared__ __align__(16) unsigned short sa[3][kRotPts * kRotLd];
+    constexpr int NPT = PTS / 16;   // 16-point tiles per block
+    __shared__ __align__(16) unsigned short sa[3][PTS * kRotLd];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, g = lane >> 4;
-    const long p0 = (long)blockIdx.x * kRotPts;
+    const long p0 = (long)blockIdx.x * PTS;
     const int ntile = (n + 15) / 16;
     const long plane = (long)n * nk;
-    f32x4_t acc[4][kRotMaxCT];
+    f32x4_t acc[NPT][kRotMaxCT];
 #pragma unroll
-    for (int pt = 0; pt < 4; pt++)
+    for (int pt = 0; pt < NPT; pt++)
 #pragma unroll
         for (int ct = 0; ct < kRotMaxCT; ct++) acc[pt][ct] = f32x4_t{0.f, 0.f, 0.f, 0.f};
     for (int k0 = 0; k0 < nk; k0 += kRotK) {
-        // stage Y[p0 .. p0+63][k0 .. k0+31]: consecutive threads on
+        // stage Y[p0 .. p0+PTS-1][k0 .. k0+31]: consecutive threads on
         // consecutive points (256-byte column segments), all loads first
         constexpr int NT = 64 * kRotWaves;
-        float v[kRotPts * kRotK / NT];
+        float v[PTS * kRotK / NT];
 #pragma unroll
-        for (int r = 0; r < kRotPts * kRotK / NT; r++) {
+        for (int r = 0; r < PTS * kRotK / NT; r++) {
             const int e = threadIdx.x + NT * r;
-            const int kk = e / kRotPts, pp = e % kRotPts;
+            const int kk = e / PTS, pp = e % PTS;
             const bool ok = k0 + kk < n && p0 + pp < nd;
             v[r] = Y[ok ? (long)(k0 + kk) * ldy + p0 + pp : 0L];
             if (!ok) v[r] = 0.f;
         }
+        // this wave's Q pieces of the chunk, one 16-column tile per ct
+        bf16x8_t bv[kRotMaxCT][3];
+#pragma unroll
+        for (int ct = 0; ct < kRotMaxCT; ct++) {
+            const int col = min((wave + kRotWaves * ct) * 16 + li, n - 1);
+#pragma unroll
+            for (int q = 0; q < 3; q++)
+                bv[ct][q] = *reinterpret_cast<const bf16x8_t *>(qp + q * plane + (long)col * nk + k0 + 8 * g);
+        }
         __syncthreads();  // the previous chunk's operands have been read
 #pragma unroll
-        for (int r = 0; r < kRotPts * kRotK / NT; r++) {
+        for (int r = 0; r < PTS * kRotK / NT; r++) {
             const int e = threadIdx.x + NT * r;
-            const int kk = e / kRotPts, pp = e % kRotPts;
+            const int kk = e / PTS, pp = e % PTS;
             unsigned short a0, a1, a2;
             split3(v[r], a0, a1, a2);
             sa[0][pp * kRotLd + kk] = a0;
@@ -349,31 +364,23 @@ __global__ __launch_bounds__(64 * kRotWaves) void rot_bf16x3(const float *__rest
             sa[2][pp * kRotLd + kk] = a2;
         }
         __syncthreads();
-        bf16x8_t av[4][3];
 #pragma unroll
-        for (int pt = 0; pt < 4; pt++)
-#pragma unroll
-            for (int q = 0; q < 3; q++)
-                av[pt][q] = *reinterpret_cast<const bf16x8_t *>(&sa[q][(pt * 16 + li) * kRotLd + 8 * g]);
-#pragma unroll
-        for (int ct = 0; ct < kRotMaxCT; ct++) {
-            const int tile = wave + kRotWaves * ct;
-            if (tile >= ntile) break;
-            const int col = min(tile * 16 + li, n - 1);
-            bf16x8_t bv[3];
+        for (int pt = 0; pt < NPT; pt++) {
+            bf16x8_t av[3];
 #pragma unroll
             for (int q = 0; q < 3; q++)
-                bv[q] = *reinterpret_cast<const bf16x8_t *>(qp + q * plane + (long)col * nk + k0 + 8 * g);
+                av[q] = *reinterpret_cast<const bf16x8_t *>(&sa[q][(pt * 16 + li) * kRotLd + 8 * g]);
 #pragma unroll
-            for (int pt = 0; pt < 4; pt++) {
+            for (int ct = 0; ct < kRotMaxCT; ct++) {
+                if (wave + kRotWaves * ct >= ntile) break;
                 // smallest products first
                 f32x4_t c = acc[pt][ct];
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[pt][1], bv[1], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[pt][0], bv[2], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[pt][2], bv[0], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[pt][0], bv[1], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[pt][1], bv[0], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[pt][0], bv[0], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[1], bv[ct][1], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[0], bv[ct][2], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[2], bv[ct][0], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[0], bv[ct][1], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[1], bv[ct][0], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[0], bv[ct][0], c, 0, 0, 0);
                 acc[pt][ct] = c;
             }
         }
@@ -385,7 +392,7 @@ __global__ __launch_bounds__(64 * kRotWaves) void rot_bf16x3(const float *__rest
         const int col = tile * 16 + li;
         if (col >= n) continue;
 #pragma unroll
-        for (int pt = 0; pt < 4; pt++) {
+        for (int pt = 0; pt < NPT; pt++) {
             const long p = p0 + pt * 16 + 4 * g;
             float *o = out + (long)col * ldo + p;
             if (p + 3 < nd && (reinterpret_cast<uintptr_t>(o) & 15) == 0) {
@@ -723,8 +730,9 @@ int rotate_device(sgpu_context *ctx, int dtype, int ns, const void *Y, long ldy,
         split_q_bf16x3<<<(unsigned)((tot + 255) / 256), 256, 0, ctx->stream>>>(
             q64, ns, nk, (unsigned short *)qd);
         SGPU_HIP(hipGetLastError());
-        rot_bf16x3<<<(unsigned)((ctx->nd + kRotPts - 1) / kRotPts), 64 * kRotWaves, 0, ctx->stream>>>(
-            (const float *)Y, ldy, (const unsigned short *)qd, ns, nk, ctx->nd, (float *)out, ldo);
+        rot_bf16x3<kRotPts><<<(unsigned)((ctx->nd + kRotPts - 1) / kRotPts), 64 * kRotWaves, 0,
+                               ctx->stream>>>((const float *)Y, ldy, (const unsigned short *)qd, ns,
+                                              nk, ctx->nd, (float *)out, ldo);
         SGPU_HIP(hipGetLastError());
     } else {
         SGPU_CHECK(sgpu_scratch(ctx, nn * sizeof(float), &qd));
