@@ -411,21 +411,27 @@ __global__ __launch_bounds__(64 * kRotWaves) void rot_bf16x3(const float *__rest
 // pieces per value (six products, as gram_bf16x3), for the large state counts
 // at which the per-block split of gram_bf16x3's 64 x 64 tiles lost to rocBLAS:
 // 128 x 128 output tiles (each staged value is split once per 128 output
-// columns), eight waves of 32 x 64, K staged 32 points at a time as
-// [piece][column][k] BF16 in LDS (16-byte operand reads).  SYM (A == B): only
-// the tiles on and above the diagonal are computed and each off-diagonal one
-// is also written transposed.  One block per (tile, K-slice); the slices are
-// summed in FP64 by sum_partials.
+// columns), eight waves of 32 x 64, K staged 64 points at a time as
+// [piece][column][k] BF16 in LDS (two MFMA K-steps per chunk, 16-byte operand
+// reads).  Each thread stages four consecutive points of a column: one
+// 16-byte load per operand and quad, one 8-byte LDS store per piece; the next
+// chunk's loads are issued before this chunk's MFMAs.  SYM (A == B): only the
+// tiles on and above the diagonal are computed and each off-diagonal one is
+// also written transposed.  One block per (tile, K-slice); the slices are
+// summed in FP64 by sum_partials.  Needs lda, ldb multiples of 4 and 16-byte
+// aligned operands (the caller checks).
 constexpr int kG3T = 128;          // output tile
-constexpr int kG3K = 32;           // K per chunk
-constexpr int kG3Ld = kG3K + 8;    // LDS row (bf16): 80 bytes
+constexpr int kG3K = 64;           // K per chunk
+constexpr int kG3Ld = kG3K + 8;    // LDS row (bf16): 144 bytes
+constexpr size_t kG3Shm = 2 * 3 * (size_t)kG3T * kG3Ld * sizeof(unsigned short);
 
 template <bool SYM>
 __global__ __launch_bounds__(512) void gram3_bf16(const float *__restrict__ A, long lda,
                                                   const float *__restrict__ B, long ldb, int n,
                                                   long nd, long kslice, float *__restrict__ part)
 {
-    __shared__ __align__(16) unsigned short sa[3][kG3T * kG3Ld], sb[3][kG3T * kG3Ld];
+    extern __shared__ __align__(16) unsigned short g3sm[];
+    unsigned short *sa = g3sm, *sb = g3sm + 3 * kG3T * kG3Ld;   // [piece][column][k]
     const int tiles = (n + kG3T - 1) / kG3T;
     // the tiles of one K-slice are consecutive logical ids -> one XCD, so
     // the slice's columns they share are L2 hits
@@ -455,21 +461,41 @@ __global__ __launch_bounds__(512) void gram3_bf16(const float *__restrict__ A, l
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 4; b++) acc[a][b] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-    // stage: element e = (column, k), k fastest (128-byte column segments);
-    // the next chunk's loads are issued before this chunk's MFMAs
-    constexpr int PER = kG3T * kG3K / 512;
-    float va[PER], vb[PER];
+    // quad e = (column, k / 4), k fastest: 16 threads cover a column's 64
+    // points (256 contiguous bytes)
+    constexpr int PER = kG3T * kG3K / 4 / 512;
+    float4 va[PER], vb[PER];
     auto load = [&](long kb) {
 #pragma unroll
         for (int r = 0; r < PER; r++) {
             const int e = threadIdx.x + 512 * r;
-            const int kk = e % kG3K, c = e / kG3K;
-            const long k = kb + kk;
+            const int kq = 4 * (e % (kG3K / 4)), c = e / (kG3K / 4);
+            const long k = kb + kq;
+            // k is a multiple of 4 below nd, so k + 3 stays inside the column's
+            // leading dimension; points at or past k1 are zeroed
             const bool oka = k < k1 && i0 + c < n, okb = k < k1 && j0 + c < n;
-            va[r] = A[oka ? (long)(i0 + c) * lda + k : 0L];
-            vb[r] = B[okb ? (long)(j0 + c) * ldb + k : 0L];
-            if (!oka) va[r] = 0.f;
-            if (!okb) vb[r] = 0.f;
+            va[r] = *reinterpret_cast<const float4 *>(A + (oka ? (long)(i0 + c) * lda + k : 0L));
+            vb[r] = *reinterpret_cast<const float4 *>(B + (okb ? (long)(j0 + c) * ldb + k : 0L));
+            const int nv = (int)min(4L, k1 - k);
+            if (!oka) va[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (!okb) vb[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (nv < 4) { va[r].w = 0.f; vb[r].w = 0.f; }
+            if (nv < 3) { va[r].z = 0.f; vb[r].z = 0.f; }
+            if (nv < 2) { va[r].y = 0.f; vb[r].y = 0.f; }
+        }
+    };
+    auto put = [&](unsigned short *dst, int o, float4 v) {
+        unsigned short p[4][3];
+        split3(v.x, p[0][0], p[0][1], p[0][2]);
+        split3(v.y, p[1][0], p[1][1], p[1][2]);
+        split3(v.z, p[2][0], p[2][1], p[2][2]);
+        split3(v.w, p[3][0], p[3][1], p[3][2]);
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const unsigned long long w = (unsigned long long)p[0][q] | ((unsigned long long)p[1][q] << 16) |
+                                         ((unsigned long long)p[2][q] << 32) |
+                                         ((unsigned long long)p[3][q] << 48);
+            *reinterpret_cast<unsigned long long *>(dst + q * kG3T * kG3Ld + o) = w;
         }
     };
     load(k0);
@@ -478,44 +504,42 @@ __global__ __launch_bounds__(512) void gram3_bf16(const float *__restrict__ A, l
 #pragma unroll
         for (int r = 0; r < PER; r++) {
             const int e = threadIdx.x + 512 * r;
-            const int kk = e % kG3K, c = e / kG3K;
-            const int o = c * kG3Ld + kk;
-            unsigned short p0, p1, p2;
-            split3(va[r], p0, p1, p2);
-            sa[0][o] = p0;
-            sa[1][o] = p1;
-            sa[2][o] = p2;
-            split3(vb[r], p0, p1, p2);
-            sb[0][o] = p0;
-            sb[1][o] = p1;
-            sb[2][o] = p2;
+            const int kq = 4 * (e % (kG3K / 4)), c = e / (kG3K / 4);
+            const int o = c * kG3Ld + kq;
+            put(sa, o, va[r]);
+            put(sb, o, vb[r]);
         }
         __syncthreads();
         if (kb + kG3K < k1) load(kb + kG3K);
-        bf16x8_t av[2][3], bv[4][3];
 #pragma unroll
-        for (int a = 0; a < 2; a++)
+        for (int ks = 0; ks < kG3K; ks += 32) {
+            bf16x8_t av[2][3], bv[4][3];
 #pragma unroll
-            for (int q = 0; q < 3; q++)
-                av[a][q] = *reinterpret_cast<const bf16x8_t *>(&sa[q][(wi + a * 16 + li) * kG3Ld + 8 * g]);
+            for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++)
+                for (int q = 0; q < 3; q++)
+                    av[a][q] = *reinterpret_cast<const bf16x8_t *>(
+                        &sa[q * kG3T * kG3Ld + (wi + a * 16 + li) * kG3Ld + ks + 8 * g]);
 #pragma unroll
-            for (int q = 0; q < 3; q++)
-                bv[b][q] = *reinterpret_cast<const bf16x8_t *>(&sb[q][(wj + b * 16 + li) * kG3Ld + 8 * g]);
+            for (int b = 0; b < 4; b++)
 #pragma unroll
-        for (int a = 0; a < 2; a++)
+                for (int q = 0; q < 3; q++)
+                    bv[b][q] = *reinterpret_cast<const bf16x8_t *>(
+                        &sb[q * kG3T * kG3Ld + (wj + b * 16 + li) * kG3Ld + ks + 8 * g]);
 #pragma unroll
-            for (int b = 0; b < 4; b++) {
-                f32x4_t c = acc[a][b];
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][1], bv[b][1], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][0], bv[b][2], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][2], bv[b][0], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][0], bv[b][1], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][1], bv[b][0], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][0], bv[b][0], c, 0, 0, 0);
-                acc[a][b] = c;
-            }
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    f32x4_t c = acc[a][b];
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][1], bv[b][1], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][0], bv[b][2], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][2], bv[b][0], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][0], bv[b][1], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][1], bv[b][0], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][0], bv[b][0], c, 0, 0, 0);
+                    acc[a][b] = c;
+                }
+        }
     }
     float *pp = part + (long)slice * n * n;
 #pragma unroll
@@ -614,7 +638,9 @@ int gram_tn_typed(sgpu_context *ctx, int n, long nd, const T *A, long lda, const
                   double *C, bool symmetric)
 {
     long kslice = gram_kslice(n, nd, sizeof(T) == 4 ? SGPU_F32 : SGPU_F64);
-    const bool g3 = sizeof(T) == 4 && n >= 256 && gram_mode() == 0;
+    const bool g3 = sizeof(T) == 4 && n >= 256 && gram_mode() == 0 && lda % 4 == 0 &&
+                    ldb % 4 == 0 && (reinterpret_cast<uintptr_t>(A) & 15) == 0 &&
+                    (reinterpret_cast<uintptr_t>(B) & 15) == 0;
     const int tiles3 = (n + kG3T - 1) / kG3T;
     const bool sym3 = symmetric || (A == B && lda == ldb);
     const unsigned nblk3 = sym3 ? (unsigned)(tiles3 * (tiles3 + 1) / 2) : (unsigned)(tiles3 * tiles3);
@@ -632,11 +658,15 @@ int gram_tn_typed(sgpu_context *ctx, int n, long nd, const T *A, long lda, const
         // 128 x 128 tiles of the gfx950 BF16-piece kernel; Mp = Y'Y computes
         // the upper triangle of tiles only
         const unsigned nblk = nblk3;
+        SGPU_HIP(hipFuncSetAttribute((const void *)gram3_bf16<true>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kG3Shm));
+        SGPU_HIP(hipFuncSetAttribute((const void *)gram3_bf16<false>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kG3Shm));
         if (sym3)
-            gram3_bf16<true><<<dim3(nblk, nslice), 512, 0, ctx->stream>>>(
+            gram3_bf16<true><<<dim3(nblk, nslice), 512, kG3Shm, ctx->stream>>>(
                 (const float *)A, lda, (const float *)B, ldb, n, nd, kslice, (float *)part);
         else
-            gram3_bf16<false><<<dim3(nblk, nslice), 512, 0, ctx->stream>>>(
+            gram3_bf16<false><<<dim3(nblk, nslice), 512, kG3Shm, ctx->stream>>>(
                 (const float *)A, lda, (const float *)B, ldb, n, nd, kslice, (float *)part);
         SGPU_HIP(hipGetLastError());
         sum_partials<T><<<(unsigned)((nn + 255) / 256), 256, 0, ctx->stream>>>(part, nslice, nn, C);
