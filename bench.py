@@ -462,6 +462,53 @@ def cpu_baseline_scf_iteration(name="si64", n_iter=2):
                       f"{p['rho_trigger']} passes) took {times[0]:.2f} s"}
 
 
+def eigensolve_bench(n=480, reps=5):
+    """The FP64 generalized eigensolve (DP_Solve_Generalized_EigenProblem) at n
+    states on a matrix pair with a Chebyshev-filtered block's spread of scales
+    (Mp diagonal over eight decades, nearly diagonal Hp), per call and per
+    phase (HIP events): blocked Cholesky + L^-1 + products, tridiagonal
+    reduction (cooperative launch), dstedc, dormtr, back-transformation; and
+    the same with rocSOLVER's dsyevd for the eigenproblem and with rocSOLVER's
+    dpotrf + triangular solves, for comparison."""
+    from sparc_amd import gpu
+    from sparc_amd.workloads import Grid
+    rng = np.random.default_rng(3)
+    d = 10.0 ** rng.uniform(-4, 4, n)
+    lam0 = np.sort(rng.uniform(-0.3, 0.5, n))
+    E = 1e-3 * rng.standard_normal((n, n))
+    Mp = np.sqrt(np.outer(d, d)) * (np.eye(n) + (E + E.T) / 2)
+    F = 1e-3 * rng.standard_normal((n, n))
+    Hp = np.sqrt(np.outer(d, d)) * (np.diag(lam0) + (F + F.T) / 2)
+    phases = ("eig_potrf", "eig_cholesky", "eig_tridiag", "eig_stedc", "eig_ormtr", "eig_back",
+              "eigensolve")
+    out = {"n": n, "reps": reps}
+    settings = {"default": {}, "dsyevd": {"SPARC_AMD_COOP_TRIDIAG": "0"},
+                "rocsolver_cholesky": {"SPARC_AMD_CHOL": "rocsolver"}}
+    saved = {k: os.environ.get(k) for k in ("SPARC_AMD_COOP_TRIDIAG", "SPARC_AMD_CHOL")}
+    try:
+        for name, env in settings.items():
+            for k in saved:
+                os.environ.pop(k, None)
+            os.environ.update(env)
+            with gpu.Context(Grid(8, 8, 8, 0.5, 0.5, 0.5)) as ctx:
+                ctx.DP_Solve_Generalized_EigenProblem(Hp, Mp)
+                ctx.timer_reset()
+                ctx.timer_enable(True)
+                for _ in range(reps):
+                    ctx.DP_Solve_Generalized_EigenProblem(Hp, Mp)
+                ctx.timer_enable(False)
+                out[name] = {k: round(ctx.timer(k)[0] / reps, 3) for k in phases}
+            print(f"[bench] eigensolve n={n} {name}: " +
+                  ", ".join(f"{k} {v:.3f} ms" for k, v in out[name].items()), flush=True)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -478,9 +525,10 @@ def main(argv=None):
     ap.add_argument("--scf", default="si8,si64,sial216",
                     help="comma-separated SCF workloads to time (sic8, si8, si64, sial216) or "
                          "'none'")
-    ap.add_argument("--only", choices=["scf", "poisson"], default=None,
-                    help="profiling aid: run only the SCF or only the Poisson comparison and "
-                         "print its JSON object (not the benchmark line)")
+    ap.add_argument("--only", choices=["scf", "poisson", "eigensolve"], default=None,
+                    help="profiling aid: run only the SCF, the Poisson comparison or the "
+                         "480-state eigensolve phases and print its JSON object (not the "
+                         "benchmark line)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one device copy of a 512-vector FP32 block (known bytes) so "
                          "that a profile holds a plain copy kernel to scale HBM counters by")
@@ -497,6 +545,9 @@ def main(argv=None):
         return 0
     if args.only == "poisson":
         print(json.dumps(poisson_bench(*train_unet())))
+        return 0
+    if args.only == "eigensolve":
+        print(json.dumps(eigensolve_bench()))
         return 0
 
     from sparc_amd import gpu
