@@ -795,7 +795,7 @@ int rotate_raw(sgpu_context *ctx, int dtype, int ns, const void *Y, long ldy, co
 // Q' Mp Q = I (LAPACKE_dsygvd itype 1 in the reference,
 // src/eigenSolver.c:1262-1375), solved as Mp = L L' (chol_panel), L^-1
 // (dtrtri), C = L^-1 Hp L^-T (two dgemm), C Z = Z Lambda (tridiagonal
-// reduction, dstedc, dormtr) and Q = L^-T Z (dgemm).  Same reduction as
+// reduction, dstedc, ormtr_lower_wave) and Q = L^-T Z (dgemm).  Same reduction as
 // dsygvd; at n = 480 on MI355X rocSOLVER's dsygst takes 16 ms of dsygvd's
 // 29 ms, its dpotrf 1.06 ms and each dtrsm 0.16-0.18 ms, against 0.02 ms for
 // a dgemm of that size.
@@ -1423,6 +1423,168 @@ static int cholesky_device(sgpu_context *ctx, int n, double *dB, rocblas_int *di
     return SGPU_OK;
 }
 
+// Sum of v over the wave, the same in every lane: DPP row shifts by 1, 2, 4
+// and 8 (an inclusive scan within each row of 16 lanes), row_bcast15 and
+// row_bcast31 across the rows, then lane 63's total broadcast by readlane.
+// Fixed order, VALU only (a __shfl_xor butterfly compiles to six dependent
+// ds_bpermute round trips through the LDS crossbar).
+__device__ __forceinline__ double dpp_f64(double v, int ctrl_sel)
+{
+    const long long b = __double_as_longlong(v);
+    int lo = (int)(b & 0xffffffff), hi = (int)(b >> 32);
+    switch (ctrl_sel) {
+        case 0: lo = __builtin_amdgcn_update_dpp(0, lo, 0x111, 0xf, 0xf, true);
+                hi = __builtin_amdgcn_update_dpp(0, hi, 0x111, 0xf, 0xf, true); break;
+        case 1: lo = __builtin_amdgcn_update_dpp(0, lo, 0x112, 0xf, 0xf, true);
+                hi = __builtin_amdgcn_update_dpp(0, hi, 0x112, 0xf, 0xf, true); break;
+        case 2: lo = __builtin_amdgcn_update_dpp(0, lo, 0x114, 0xf, 0xf, true);
+                hi = __builtin_amdgcn_update_dpp(0, hi, 0x114, 0xf, 0xf, true); break;
+        case 3: lo = __builtin_amdgcn_update_dpp(0, lo, 0x118, 0xf, 0xf, true);
+                hi = __builtin_amdgcn_update_dpp(0, hi, 0x118, 0xf, 0xf, true); break;
+        case 4: lo = __builtin_amdgcn_update_dpp(0, lo, 0x142, 0xa, 0xf, false);
+                hi = __builtin_amdgcn_update_dpp(0, hi, 0x142, 0xa, 0xf, false); break;
+        default: lo = __builtin_amdgcn_update_dpp(0, lo, 0x143, 0xc, 0xf, false);
+                 hi = __builtin_amdgcn_update_dpp(0, hi, 0x143, 0xc, 0xf, false); break;
+    }
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <int CW>
+__device__ __forceinline__ void wave_sum_f64(double (&d)[CW])
+{
+#pragma unroll
+    for (int s = 0; s < 6; s++)
+#pragma unroll
+        for (int j = 0; j < CW; j++) d[j] += dpp_f64(d[j], s);
+#pragma unroll
+    for (int j = 0; j < CW; j++) d[j] = readlane_f64(d[j], 63);
+}
+
+// Q1 Z for the tridiagonal reduction's reflectors (LAPACK dormtr with
+// SIDE = 'L', UPLO = 'L', TRANS = 'N', as rocSOLVER's, which takes 0.9 ms at
+// n = 480 in short launches): Z <- H(0) H(1) ... H(n-2) Z with
+// H(i) = I - tau(i) v v', v(i+1) = 1, v(i+2:n) = A(i+2:n, i).  Columns of Z
+// are independent: each of the four waves of a block owns CW of them, its
+// rows spread over the lanes (row lane + 64 q, q < NR) and held in registers,
+// and applies the reflectors from the last to the first (v'z by the DPP
+// wave sum).  The reflectors are staged in LDS KC at a time, double
+// buffered: the next chunk's loads are in flight while the waves apply the
+// current one, one barrier per chunk.
+template <int NR, int CW, int KC>
+__global__ __launch_bounds__(256) void ormtr_lower_blk(const double *__restrict__ A, int lda,
+                                                       const double *__restrict__ tau,
+                                                       double *__restrict__ Z, int ldz, int n)
+{
+    extern __shared__ double osm[];
+    double *vs = osm;                       // [2][KC][n] reflector vectors
+    double *ts = osm + 2 * (size_t)KC * n;  // [2][KC] tau
+    const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
+    const int c0 = (blockIdx.x * 4 + wv) * CW;
+    constexpr int SPT = (KC * 64 * NR + 255) / 256;   // staged values per thread (n <= 64 NR)
+    double z[NR][CW];
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+        const int r = ln + 64 * q;
+#pragma unroll
+        for (int j = 0; j < CW; j++) z[q][j] = r < n && c0 + j < n ? Z[(long)(c0 + j) * ldz + r] : 0.0;
+    }
+    // chunk c holds reflectors n-2 - c KC - g, g < KC (zero past the first)
+    double st[SPT], stt = 0.0;
+    auto load_chunk = [&](int c) {
+#pragma unroll
+        for (int k = 0; k < SPT; k++) {
+            const int e = t + 256 * k;
+            const int g = e / n, r = e - g * n;
+            const int i = n - 2 - c * KC - g;
+            double v = 0.0;
+            if (g < KC && i >= 0) v = r == i + 1 ? 1.0 : (r > i + 1 ? A[(long)i * lda + r] : 0.0);
+            st[k] = v;
+        }
+        if (t < KC) {
+            const int i = n - 2 - c * KC - t;
+            stt = i >= 0 ? tau[i] : 0.0;
+        }
+    };
+    auto store_chunk = [&](int buf) {
+#pragma unroll
+        for (int k = 0; k < SPT; k++) {
+            const int e = t + 256 * k;
+            if (e < KC * n) vs[(size_t)buf * KC * n + e] = st[k];
+        }
+        if (t < KC) ts[buf * KC + t] = stt;
+    };
+    const int nchunk = (n - 1 + KC - 1) / KC;
+    load_chunk(0);
+    store_chunk(0);
+    __syncthreads();
+    for (int c = 0; c < nchunk; c++) {
+        const int buf = c & 1;
+        if (c + 1 < nchunk) load_chunk(c + 1);
+        const double *vb = vs + (size_t)buf * KC * n;
+        for (int g = 0; g < KC; g++) {
+            if (n - 2 - c * KC - g < 0) break;
+            const double ti = ts[buf * KC + g];
+            double v[NR];
+#pragma unroll
+            for (int q = 0; q < NR; q++) {
+                const int r = ln + 64 * q;
+                v[q] = r < n ? vb[(size_t)g * n + r] : 0.0;
+            }
+            double d[CW];
+#pragma unroll
+            for (int j = 0; j < CW; j++) {
+                d[j] = 0.0;
+#pragma unroll
+                for (int q = 0; q < NR; q++) d[j] += v[q] * z[q][j];
+            }
+            wave_sum_f64<CW>(d);
+#pragma unroll
+            for (int j = 0; j < CW; j++) {
+                const double w = ti * d[j];
+#pragma unroll
+                for (int q = 0; q < NR; q++) z[q][j] -= v[q] * w;
+            }
+        }
+        if (c + 1 < nchunk) store_chunk(buf ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < NR; q++) {
+        const int r = ln + 64 * q;
+#pragma unroll
+        for (int j = 0; j < CW; j++)
+            if (r < n && c0 + j < n) Z[(long)(c0 + j) * ldz + r] = z[q][j];
+    }
+}
+
+template <int NR, int CW, int KC>
+static int launch_ormtr_blk(sgpu_context *ctx, int n, const double *dA, const double *dtau,
+                            double *dZ)
+{
+    const size_t shm = (2 * (size_t)KC * n + 2 * KC) * sizeof(double);
+    SGPU_HIP(hipFuncSetAttribute((const void *)ormtr_lower_blk<NR, CW, KC>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    const unsigned nblk = (unsigned)((n + 4 * CW - 1) / (4 * CW));
+    ormtr_lower_blk<NR, CW, KC><<<nblk, 256, shm, ctx->stream>>>(dA, n, dtau, dZ, n, n);
+    SGPU_HIP(hipGetLastError());
+    return SGPU_OK;
+}
+
+static int ormtr_lower_device(sgpu_context *ctx, int n, const double *dA, const double *dtau,
+                              double *dZ)
+{
+    // rows per lane NR, columns per wave CW, reflectors per chunk KC: z and
+    // the staged chunk fit in registers, two chunks in LDS
+    const int nr = (n + 63) / 64;
+    if (nr <= 4) return launch_ormtr_blk<4, 4, 16>(ctx, n, dA, dtau, dZ);
+    if (nr <= 8) return launch_ormtr_blk<8, 4, 16>(ctx, n, dA, dtau, dZ);
+    if (nr <= 12) return launch_ormtr_blk<12, 4, 8>(ctx, n, dA, dtau, dZ);
+    if (nr <= 16) return launch_ormtr_blk<16, 4, 8>(ctx, n, dA, dtau, dZ);
+    if (nr <= 24) return launch_ormtr_blk<24, 2, 4>(ctx, n, dA, dtau, dZ);
+    sgpu_set_error("eigensolve: back-transformation limited to %d states", 24 * 64);
+    return SGPU_ERR_ARG;
+}
+
 // Workgroup count and dynamic LDS of sytrd_lower_coop for n (0 groups when
 // it does not apply): at least the requested count, more where the rows
 // would not fit in 150 KB of LDS per workgroup, at most one group per row.
@@ -1548,8 +1710,11 @@ int eigensolve_device(sgpu_context *ctx, int n, double *dA, double *dB, double *
         SGPU_CHECK(sgpu_tend(ctx, &tp, "eig_stedc"));
         SGPU_CHECK(sgpu_tbegin(ctx, &tp));
         SGPU_CHECK(sgpu_tbegin(ctx, &tq));
-        SGPU_BLAS(rocsolver_dormtr(ctx->blas, rocblas_side_left, rocblas_fill_lower,
-                                   rocblas_operation_none, n, n, dA, n, dtau, dC, n));
+        if (ctx->own_ormtr && n <= 24 * 64)
+            SGPU_CHECK(ormtr_lower_device(ctx, n, dA, dtau, dC));
+        else
+            SGPU_BLAS(rocsolver_dormtr(ctx->blas, rocblas_side_left, rocblas_fill_lower,
+                                       rocblas_operation_none, n, n, dA, n, dtau, dC, n));
         SGPU_CHECK(sgpu_tend(ctx, &tq, "eig_ormtr"));
         if (ctx->own_chol) {
             SGPU_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none,

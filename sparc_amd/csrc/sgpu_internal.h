@@ -150,6 +150,9 @@ struct sgpu_context {
     // with C = L^-1 Hp L^-T and Q = L^-T Z as products with L^-1
     // (SPARC_AMD_CHOL=rocsolver keeps dpotrf and the triangular solves)
     bool own_chol = true;
+    // the back-transformation Q1 Z by ormtr_lower_wave (SPARC_AMD_ORMTR=rocsolver
+    // keeps rocSOLVER's dormtr)
+    bool own_ormtr = true;
     int n_cu = 256;   // compute units of the device (set by sgpu_create)
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;

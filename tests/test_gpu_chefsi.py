@@ -390,3 +390,33 @@ def test_generalized_eigensolve_blocked_cholesky(gpu, monkeypatch, n):
             ctx.DP_Solve_Generalized_EigenProblem(np.eye(n), bad)
         lam2, _ = ctx.DP_Solve_Generalized_EigenProblem(Hp, Mp)   # usable afterwards
     assert np.max(np.abs(lam2 - lam_ref)) < 1e-10
+
+
+@pytest.mark.parametrize("n", [24, 181, 480, 1000])
+def test_generalized_eigensolve_back_transformation(gpu, monkeypatch, n):
+    """The back-transformation Q1 Z by one wave per four columns of Z (the
+    reflectors applied from the last to the first in registers) against
+    rocSOLVER's dormtr (SPARC_AMD_ORMTR=rocsolver): the same eigenvalues and
+    eigenvectors to rounding (same tridiagonal matrix and dstedc), and both
+    Mp-orthonormal and diagonalizing Hp; n = 24 takes the one-workgroup
+    reduction, the others the cooperative one."""
+    from sparc_amd.workloads import Grid
+    rng = np.random.default_rng(200 + n)
+    d = 10.0 ** rng.uniform(-3, 3, n)
+    lam0 = np.sort(rng.uniform(-0.3, 0.5, n))
+    E = 1e-3 * rng.standard_normal((n, n))
+    Mp = np.sqrt(np.outer(d, d)) * (np.eye(n) + (E + E.T) / 2)
+    F = 1e-2 * rng.standard_normal((n, n))
+    Hp = np.sqrt(np.outer(d, d)) * (np.diag(lam0) + (F + F.T) / 2)
+    g = Grid(8, 8, 8, 0.5, 0.5, 0.5)
+    out = {}
+    for flag in ("own", "rocsolver"):
+        monkeypatch.setenv("SPARC_AMD_ORMTR", flag)
+        with gpu.Context(g) as ctx:
+            out[flag] = ctx.DP_Solve_Generalized_EigenProblem(Hp, Mp)
+    (lam, Q), (lam_r, Q_r) = out["own"], out["rocsolver"]
+    assert np.array_equal(lam, lam_r)
+    scale = np.max(np.abs(Q_r))
+    assert np.max(np.abs(Q - Q_r)) < 1e-11 * scale
+    assert np.max(np.abs(Q.T @ Mp @ Q - np.eye(n))) < 1e-9
+    assert np.max(np.abs(Q.T @ Hp @ Q - np.diag(lam))) < 1e-9
