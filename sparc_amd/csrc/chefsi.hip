@@ -799,6 +799,51 @@ int rotate_raw(sgpu_context *ctx, int dtype, int ns, const void *Y, long ldy, co
 // dsygvd; at n = 480 on MI355X rocSOLVER's dsygst takes 16 ms of dsygvd's
 // 29 ms, its dpotrf 1.06 ms and each dtrsm 0.16-0.18 ms, against 0.02 ms for
 // a dgemm of that size.
+__device__ __forceinline__ double readlane_f64(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum of v over the wave, the same in every lane: DPP row shifts by 1, 2, 4
+// and 8 (an inclusive scan within each row of 16 lanes), row_bcast15 and
+// row_bcast31 across the rows, then lane 63's total broadcast by readlane.
+// Fixed order, VALU only (a __shfl_xor butterfly compiles to six dependent
+// ds_bpermute round trips through the LDS crossbar).
+__device__ __forceinline__ double dpp_f64(double v, int ctrl_sel)
+{
+    const long long b = __double_as_longlong(v);
+    int lo = (int)(b & 0xffffffff), hi = (int)(b >> 32);
+    switch (ctrl_sel) {
+        case 0: lo = __builtin_amdgcn_update_dpp(0, lo, 0x111, 0xf, 0xf, true);
+                hi = __builtin_amdgcn_update_dpp(0, hi, 0x111, 0xf, 0xf, true); break;
+        case 1: lo = __builtin_amdgcn_update_dpp(0, lo, 0x112, 0xf, 0xf, true);
+                hi = __builtin_amdgcn_update_dpp(0, hi, 0x112, 0xf, 0xf, true); break;
+        case 2: lo = __builtin_amdgcn_update_dpp(0, lo, 0x114, 0xf, 0xf, true);
+                hi = __builtin_amdgcn_update_dpp(0, hi, 0x114, 0xf, 0xf, true); break;
+        case 3: lo = __builtin_amdgcn_update_dpp(0, lo, 0x118, 0xf, 0xf, true);
+                hi = __builtin_amdgcn_update_dpp(0, hi, 0x118, 0xf, 0xf, true); break;
+        case 4: lo = __builtin_amdgcn_update_dpp(0, lo, 0x142, 0xa, 0xf, false);
+                hi = __builtin_amdgcn_update_dpp(0, hi, 0x142, 0xa, 0xf, false); break;
+        default: lo = __builtin_amdgcn_update_dpp(0, lo, 0x143, 0xc, 0xf, false);
+                 hi = __builtin_amdgcn_update_dpp(0, hi, 0x143, 0xc, 0xf, false); break;
+    }
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <int CW>
+__device__ __forceinline__ void wave_sum_f64(double (&d)[CW])
+{
+#pragma unroll
+    for (int s = 0; s < 6; s++)
+#pragma unroll
+        for (int j = 0; j < CW; j++) d[j] += dpp_f64(d[j], s);
+#pragma unroll
+    for (int j = 0; j < CW; j++) d[j] = readlane_f64(d[j], 63);
+}
+
 // Householder reduction of a symmetric matrix to tridiagonal form inside one
 // workgroup, the matrix held in LDS: LAPACK dsytd2 with UPLO = 'L' (the
 // storage rocsolver_dormtr reads back): for i = 0..n-2, H(i) = I - tau v v'
@@ -815,12 +860,13 @@ constexpr int kTridThreads = 1024;
 
 __device__ __forceinline__ int tri_at(int n, int r, int c) { return c * (2 * n - c + 1) / 2 + (r - c); }
 
-// Sum over the workgroup in a fixed order (wave shuffles, then the 16 wave
+// Sum over the workgroup in a fixed order (DPP wave sums, then the 16 wave
 // sums in order); the total is valid in thread 0.
 __device__ __forceinline__ double tri_block_sum(double v, double *red)
 {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    double a[1] = {v};
+    wave_sum_f64<1>(a);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a[0];
     __syncthreads();
     double s = 0.0;
     if (threadIdx.x == 0)
@@ -1006,9 +1052,10 @@ __device__ __forceinline__ bool coop_barrier(CoopSync *sy, unsigned int target, 
 template <int NT>
 __device__ __forceinline__ double coop_block_sum(double v, double *red)
 {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    double a[1] = {v};
+    wave_sum_f64<1>(a);
     __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a[0];
     __syncthreads();
     double s = 0.0;
     for (int k = 0; k < NT / 64; k++) s += red[k];
@@ -1080,7 +1127,9 @@ __global__ __launch_bounds__(NT) void sytrd_lower_coop(double *__restrict__ A, i
             const double *row = Ar + (size_t)k * ldr + 1;
             double s = 0.0;
             for (int c = ln; c < n - 1; c += 64) s += row[c] * v[c];
-            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+            double a[1] = {s};
+            wave_sum_f64<1>(a);
+            s = a[0];
             if (ln == 0) pown[k] = s;
             dsum += s * v[g + k * G - 1];
         }
@@ -1126,7 +1175,11 @@ __global__ __launch_bounds__(NT) void sytrd_lower_coop(double *__restrict__ A, i
         const double p0 = coop_load(pb), r0 = coop_load(pb + n);
         double pv = ln < G ? coop_load(pb + 2 * n + ln) : 0.0;
         if (ln + 64 < G) pv += coop_load(pb + 2 * n + 64 + ln);
-        for (int off = 32; off > 0; off >>= 1) pv += __shfl_xor(pv, off, 64);
+        {
+            double a[1] = {pv};
+            wave_sum_f64<1>(a);
+            pv = a[0];
+        }
         const double a2 = -0.5 * ti * (ti * pv);
         // w = tau p + a2 v; the next column x = row(i+1) - v(0) w - w(0) v
         // below its diagonal (v(0) = 1), with the squares of x(1:) summed
@@ -1147,7 +1200,11 @@ __global__ __launch_bounds__(NT) void sytrd_lower_coop(double *__restrict__ A, i
             }
         }
         if (m == 1) break;
-        for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
+        {
+            double a[1] = {ss};
+            wave_sum_f64<1>(a);
+            ss = a[0];
+        }
         if (ln == 0) red[wv] = ss;
         __syncthreads();
         // Householder vector of x (dlarfg), the same bits in every workgroup
@@ -1209,9 +1266,7 @@ __global__ __launch_bounds__(NT) void sytrd_lower_coop(double *__restrict__ A, i
                             if (h1) row[u][c1] = a1[u];
                         }
                 }
-                for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-                    for (int u = 0; u < 4; u++) acc[u] += __shfl_xor(acc[u], off, 64);
+                wave_sum_f64<4>(acc);
 #pragma unroll
                 for (int u = 0; u < 4; u++)
                     if (u < nu) {
@@ -1261,13 +1316,6 @@ constexpr int kCholLd = kCholNB + 1;
 constexpr size_t kCholShm = (4 * 2 + 3) * (size_t)kCholNB * kCholLd * sizeof(double) +
                             kCholNB * sizeof(double);
 
-__device__ __forceinline__ double readlane_f64(double v, int lane)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), lane);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
 
 __global__ __launch_bounds__(kCholThreads) void chol_panel(double *__restrict__ B, int n, int kb,
                                                            int *__restrict__ info)
@@ -1421,43 +1469,6 @@ static int cholesky_device(sgpu_context *ctx, int n, double *dB, rocblas_int *di
         SGPU_HIP(hipGetLastError());
     }
     return SGPU_OK;
-}
-
-// Sum of v over the wave, the same in every lane: DPP row shifts by 1, 2, 4
-// and 8 (an inclusive scan within each row of 16 lanes), row_bcast15 and
-// row_bcast31 across the rows, then lane 63's total broadcast by readlane.
-// Fixed order, VALU only (a __shfl_xor butterfly compiles to six dependent
-// ds_bpermute round trips through the LDS crossbar).
-__device__ __forceinline__ double dpp_f64(double v, int ctrl_sel)
-{
-    const long long b = __double_as_longlong(v);
-    int lo = (int)(b & 0xffffffff), hi = (int)(b >> 32);
-    switch (ctrl_sel) {
-        case 0: lo = __builtin_amdgcn_update_dpp(0, lo, 0x111, 0xf, 0xf, true);
-                hi = __builtin_amdgcn_update_dpp(0, hi, 0x111, 0xf, 0xf, true); break;
-        case 1: lo = __builtin_amdgcn_update_dpp(0, lo, 0x112, 0xf, 0xf, true);
-                hi = __builtin_amdgcn_update_dpp(0, hi, 0x112, 0xf, 0xf, true); break;
-        case 2: lo = __builtin_amdgcn_update_dpp(0, lo, 0x114, 0xf, 0xf, true);
-                hi = __builtin_amdgcn_update_dpp(0, hi, 0x114, 0xf, 0xf, true); break;
-        case 3: lo = __builtin_amdgcn_update_dpp(0, lo, 0x118, 0xf, 0xf, true);
-                hi = __builtin_amdgcn_update_dpp(0, hi, 0x118, 0xf, 0xf, true); break;
-        case 4: lo = __builtin_amdgcn_update_dpp(0, lo, 0x142, 0xa, 0xf, false);
-                hi = __builtin_amdgcn_update_dpp(0, hi, 0x142, 0xa, 0xf, false); break;
-        default: lo = __builtin_amdgcn_update_dpp(0, lo, 0x143, 0xc, 0xf, false);
-                 hi = __builtin_amdgcn_update_dpp(0, hi, 0x143, 0xc, 0xf, false); break;
-    }
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
-template <int CW>
-__device__ __forceinline__ void wave_sum_f64(double (&d)[CW])
-{
-#pragma unroll
-    for (int s = 0; s < 6; s++)
-#pragma unroll
-        for (int j = 0; j < CW; j++) d[j] += dpp_f64(d[j], s);
-#pragma unroll
-    for (int j = 0; j < CW; j++) d[j] = readlane_f64(d[j], 63);
 }
 
 // Q1 Z for the tridiagonal reduction's reflectors (LAPACK dormtr with
