@@ -1302,9 +1302,9 @@ __global__ __launch_bounds__(NT) void sytrd_lower_coop(double *__restrict__ A, i
 // of a launch owns the diagonal block, block b >= 1 the 32 rows
 // kb + nb + 32 (b - 1) ... below it.  Every block forms the diagonal block
 // D = A11 - L10 L10' (the same bits in every block) and, for a row block,
-// P = A21 - L20 L10': the columns of L10 / L20 are staged four chunks of 32
-// at a time (one chunk per wave, the four partial sums added in a fixed
-// order).  One wave factors D in registers (rows in lanes, the pivot column
+// P = A21 - L20 L10' on FP64 matrix cores: the columns of L10 / L20 are
+// staged four chunks of 32 at a time (one chunk per wave, the four partial
+// sums added in a fixed order).  One wave factors D in registers (rows in lanes, the pivot column
 // broadcast with readlane); the row blocks then solve L21 L11' = P row by
 // row.  Block 0 also zeroes the strict upper triangle of the panel's columns
 // (the factor is then used as a full matrix).  info gets the 1-based column
@@ -1333,9 +1333,20 @@ __global__ __launch_bounds__(kCholThreads) void chol_panel(double *__restrict__ 
     const int b = blockIdx.x;
     const int r0 = kb + nb + kCholNB * (b - 1);
     const int nr = b == 0 ? 0 : min(kCholNB, n - r0);
-    // lane tile: rows ri + 8 a, columns cj + 8 c (a, c < 4)
-    const int ri = ln >> 3, cj = ln & 7;
-    double ad[4][4] = {}, ap[4][4] = {};
+    // D and P of this wave's chunks on v_mfma_f64_16x16x4f64: 16 x 16 tiles
+    // (a, c) of D = Ld Ld' and P = Lr Ld'; operand lane (li, lk) holds row li
+    // of a tile at k = 4 s + lk, which for D's B operand is again Ld (row j).
+    // D(i, j) of a tile ends in lane j + 16 (i % 4), register i / 4.
+    const int li = ln & 15, lk = ln >> 4;
+    typedef double f64x4_t __attribute__((ext_vector_type(4)));
+    f64x4_t ad[2][2], ap[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            ad[a][c] = f64x4_t{0.0, 0.0, 0.0, 0.0};
+            ap[a][c] = f64x4_t{0.0, 0.0, 0.0, 0.0};
+        }
     for (int s0 = 0; s0 < kb; s0 += 4 * kCholNB) {
         const int nch = min(4, (kb - s0) / kCholNB);
         for (int e = t; e < nch * kCholNB * kCholNB; e += kCholThreads) {
@@ -1348,22 +1359,19 @@ __global__ __launch_bounds__(kCholThreads) void chol_panel(double *__restrict__ 
         if (wv < nch) {
             const double(*Ld)[kCholLd] = St + (2 * wv) * kCholNB;
             const double(*Lr)[kCholLd] = St + (2 * wv + 1) * kCholNB;
-#pragma unroll 4
-            for (int kk = 0; kk < kCholNB; kk++) {
-                double li[4], lr[4], lj[4];
 #pragma unroll
-                for (int a = 0; a < 4; a++) {
-                    li[a] = Ld[ri + 8 * a][kk];
-                    lr[a] = Lr[ri + 8 * a][kk];
-                    lj[a] = Ld[cj + 8 * a][kk];
-                }
-#pragma unroll
-                for (int a = 0; a < 4; a++)
-#pragma unroll
-                    for (int c = 0; c < 4; c++) {
-                        ad[a][c] += li[a] * lj[c];
-                        ap[a][c] += lr[a] * lj[c];
-                    }
+            for (int s = 0; s < kCholNB / 4; s++) {
+                const int k = 4 * s + lk;
+                const double d0 = Ld[li][k], d1 = Ld[16 + li][k];
+                const double p0 = Lr[li][k], p1 = Lr[16 + li][k];
+                ad[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(d0, d0, ad[0][0], 0, 0, 0);
+                ad[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(d0, d1, ad[0][1], 0, 0, 0);
+                ad[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(d1, d0, ad[1][0], 0, 0, 0);
+                ad[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(d1, d1, ad[1][1], 0, 0, 0);
+                ap[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(p0, d0, ap[0][0], 0, 0, 0);
+                ap[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(p0, d1, ap[0][1], 0, 0, 0);
+                ap[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(p1, d0, ap[1][0], 0, 0, 0);
+                ap[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(p1, d1, ap[1][1], 0, 0, 0);
             }
         }
         __syncthreads();
@@ -1371,12 +1379,15 @@ __global__ __launch_bounds__(kCholThreads) void chol_panel(double *__restrict__ 
     // partial sums of the four waves, added in wave order
     if (kb > 0) {
 #pragma unroll
-        for (int a = 0; a < 4; a++)
+        for (int a = 0; a < 2; a++)
 #pragma unroll
-            for (int c = 0; c < 4; c++) {
-                St[(2 * wv) * kCholNB + ri + 8 * a][cj + 8 * c] = ad[a][c];
-                St[(2 * wv + 1) * kCholNB + ri + 8 * a][cj + 8 * c] = ap[a][c];
-            }
+            for (int c = 0; c < 2; c++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int i = 16 * a + lk + 4 * r, j = 16 * c + li;
+                    St[(2 * wv) * kCholNB + i][j] = ad[a][c][r];
+                    St[(2 * wv + 1) * kCholNB + i][j] = ap[a][c][r];
+                }
         __syncthreads();
     }
     for (int e = t; e < kCholNB * kCholNB; e += kCholThreads) {
