@@ -1306,8 +1306,9 @@ __global__ __launch_bounds__(NT) void sytrd_lower_coop(double *__restrict__ A, i
 // staged four chunks of 32 at a time (one chunk per wave, the four partial
 // sums added in a fixed order).  One wave factors D in registers (rows in lanes, the pivot column
 // broadcast with readlane); the row blocks then solve L21 L11' = P row by
-// row.  Block 0 also zeroes the strict upper triangle of the panel's columns
-// (the factor is then used as a full matrix).  info gets the 1-based column
+// row.  Block 0 keeps L11 in a staging buffer (chol_put_diag copies the
+// diagonal blocks into place after the last panel) and zeroes the strict
+// upper triangle above it (the factor is then used as a full matrix).  info gets the 1-based column
 // of the first pivot that is not positive, and a launch after a failed one
 // returns at once.
 constexpr int kCholNB = 32;
@@ -1318,7 +1319,8 @@ constexpr size_t kCholShm = (4 * 2 + 3) * (size_t)kCholNB * kCholLd * sizeof(dou
 
 
 __global__ __launch_bounds__(kCholThreads) void chol_panel(double *__restrict__ B, int n, int kb,
-                                                           int *__restrict__ info)
+                                                           int *__restrict__ info,
+                                                           double *__restrict__ l11buf)
 {
     extern __shared__ double csm[];
     double(*St)[kCholLd] = (double(*)[kCholLd])csm;             // [4 chunks][Ld; Lr][32] rows
@@ -1440,9 +1442,13 @@ __global__ __launch_bounds__(kCholThreads) void chol_panel(double *__restrict__ 
         return;
     }
     if (b == 0) {
+        // L11 goes to l11buf, not into B: the row blocks of this launch may
+        // still be reading A11 there (chol_put_diag copies it after the
+        // last panel; later panels never read these rows)
+        double *lb = l11buf + (size_t)(kb / kCholNB) * kCholNB * kCholNB;
         for (int e = t; e < kCholNB * kCholNB; e += kCholThreads) {
             const int ii = e & 31, jj = e >> 5;
-            if (ii < nb && jj < nb) B[(long)(kb + jj) * n + kb + ii] = L11[ii][jj];
+            lb[e] = L11[ii][jj];
         }
         for (int jj = 0; jj < nb; jj++)
             for (int r = t; r < kb; r += kCholThreads) B[(long)(kb + jj) * n + r] = 0.0;
@@ -1468,7 +1474,22 @@ __global__ __launch_bounds__(kCholThreads) void chol_panel(double *__restrict__ 
     }
 }
 
-static int cholesky_device(sgpu_context *ctx, int n, double *dB, rocblas_int *dinfo)
+// The diagonal blocks L11 of every panel from l11buf into B (zeros above the
+// diagonal).
+__global__ __launch_bounds__(256) void chol_put_diag(double *__restrict__ B, int n,
+                                                     const double *__restrict__ l11buf)
+{
+    const int kb = blockIdx.x * kCholNB, nb = min(kCholNB, n - kb);
+    const double *lb = l11buf + (size_t)blockIdx.x * kCholNB * kCholNB;
+    for (int e = threadIdx.x; e < kCholNB * kCholNB; e += 256) {
+        const int ii = e & 31, jj = e >> 5;
+        if (ii < nb && jj < nb) B[(long)(kb + jj) * n + kb + ii] = lb[e];
+    }
+}
+
+// l11buf: ceil(n / 32) * 32 * 32 doubles.
+static int cholesky_device(sgpu_context *ctx, int n, double *dB, rocblas_int *dinfo,
+                           double *l11buf)
 {
     SGPU_HIP(hipFuncSetAttribute((const void *)chol_panel,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)kCholShm));
@@ -1476,9 +1497,11 @@ static int cholesky_device(sgpu_context *ctx, int n, double *dB, rocblas_int *di
     for (int kb = 0; kb < n; kb += kCholNB) {
         const int nb = std::min(kCholNB, n - kb);
         const int nblk = 1 + (n - kb - nb + kCholNB - 1) / kCholNB;
-        chol_panel<<<nblk, kCholThreads, kCholShm, ctx->stream>>>(dB, n, kb, (int *)dinfo);
+        chol_panel<<<nblk, kCholThreads, kCholShm, ctx->stream>>>(dB, n, kb, (int *)dinfo, l11buf);
         SGPU_HIP(hipGetLastError());
     }
+    chol_put_diag<<<(unsigned)((n + kCholNB - 1) / kCholNB), 256, 0, ctx->stream>>>(dB, n, l11buf);
+    SGPU_HIP(hipGetLastError());
     return SGPU_OK;
 }
 
@@ -1651,12 +1674,14 @@ int eigensolve_device(sgpu_context *ctx, int n, double *dA, double *dB, double *
     // exchange buffers, its barrier and info in the shared scratch (free again
     // once lambda is on the host)
     const size_t npub = 2 * (2 * (size_t)n + kCoopMaxG);
-    SGPU_CHECK(sgpu_scratch(ctx, (3 * (size_t)n + (size_t)n * n + npub) * sizeof(double) +
+    const size_t nl11 = (size_t)((n + kCholNB - 1) / kCholNB) * kCholNB * kCholNB;
+    SGPU_CHECK(sgpu_scratch(ctx, (3 * (size_t)n + (size_t)n * n + npub + nl11) * sizeof(double) +
                                      sizeof(CoopSync) + 64,
                             &buf));
     double *dD = (double *)buf, *dE = dD + n;
     double *pub = dE + 2 * n + (size_t)n * n;
-    CoopSync *sy = (CoopSync *)(pub + npub);
+    double *l11buf = pub + npub;
+    CoopSync *sy = (CoopSync *)(l11buf + nl11);
     rocblas_int *dinfo = (rocblas_int *)(sy + 1);
     sgpu_tmark t, tp;
     SGPU_CHECK(sgpu_tbegin(ctx, &t));
@@ -1666,7 +1691,7 @@ int eigensolve_device(sgpu_context *ctx, int n, double *dA, double *dB, double *
     sgpu_tmark tq;
     SGPU_CHECK(sgpu_tbegin(ctx, &tq));
     if (ctx->own_chol)
-        SGPU_CHECK(cholesky_device(ctx, n, dB, dinfo));
+        SGPU_CHECK(cholesky_device(ctx, n, dB, dinfo, l11buf));
     else
         SGPU_BLAS(rocsolver_dpotrf(ctx->blas, rocblas_fill_upper, n, dB, n, dinfo));
     SGPU_CHECK(sgpu_tend(ctx, &tq, "eig_potrf"));

@@ -420,3 +420,25 @@ def test_generalized_eigensolve_back_transformation(gpu, monkeypatch, n):
     assert np.max(np.abs(Q - Q_r)) < 1e-11 * scale
     assert np.max(np.abs(Q.T @ Mp @ Q - np.eye(n))) < 1e-9
     assert np.max(np.abs(Q.T @ Hp @ Q - np.diag(lam))) < 1e-9
+
+
+@pytest.mark.parametrize("n", [1200, 1600])
+def test_generalized_eigensolve_large(gpu, n):
+    """n = 1200: the cooperative reduction with more workgroups than asked
+    (its rows need about 110 workgroups' LDS); n = 1600: past what 128
+    workgroups' LDS holds, the reduction falls back to rocSOLVER's dsyevd.
+    Eigenvalues against LAPACK dsygvd, Mp-orthonormal eigenvectors."""
+    from sparc_amd.workloads import Grid
+    rng = np.random.default_rng(300 + n)
+    d = 10.0 ** rng.uniform(-2, 2, n)
+    lam0 = np.sort(rng.uniform(-0.3, 0.5, n))
+    E = 1e-3 * rng.standard_normal((n, n))
+    Mp = np.sqrt(np.outer(d, d)) * (np.eye(n) + (E + E.T) / 2)
+    F = 1e-3 * rng.standard_normal((n, n))
+    Hp = np.sqrt(np.outer(d, d)) * (np.diag(lam0) + (F + F.T) / 2)
+    with gpu.Context(Grid(8, 8, 8, 0.5, 0.5, 0.5)) as ctx:
+        lam, Q = ctx.DP_Solve_Generalized_EigenProblem(Hp, Mp)
+    lam_ref = sla.eigh(Hp, Mp, eigvals_only=True, driver="gvd")
+    assert np.max(np.abs(lam - lam_ref)) < 1e-10
+    assert np.max(np.abs(Q.T @ Mp @ Q - np.eye(n))) < 1e-9
+    assert np.max(np.abs(Q.T @ Hp @ Q - np.diag(lam))) < 1e-9
