@@ -401,8 +401,19 @@ __global__ void nlb_reduce(const T *__restrict__ part, int ncol, int n_alpha,
     if (t >= (long)n_alpha * ncol) return;
     const int r = (int)(t / ncol);
     const int col = (int)(t - (long)r * ncol);
+    // in the order of arow_src, the loads issued eight at a time (the row's
+    // ~100 partials are otherwise one load latency each)
     T s = T(0);
-    for (int e = arow_ptr[r]; e < arow_ptr[r + 1]; e++) s += part[(long)arow_src[e] * ncol + col];
+    const int e1 = arow_ptr[r + 1];
+    int e = arow_ptr[r];
+    for (; e + 8 <= e1; e += 8) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = part[(long)arow_src[e + u] * ncol + col];
+#pragma unroll
+        for (int u = 0; u < 8; u++) s += v[u];
+    }
+    for (; e < e1; e++) s += part[(long)arow_src[e] * ncol + col];
     alpha_s[(long)r * ncol + col] = (T)(scale * (gamma ? gamma[r] : 1.0)) * s;
 }
 
