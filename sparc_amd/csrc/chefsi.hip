@@ -124,8 +124,17 @@ __global__ void sum_partials(const P *__restrict__ part, int nslice, long nn,
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nn) return;
+    // slices in order, the loads issued eight at a time
     double s = 0.0;
-    for (int k = 0; k < nslice; k++) s += (double)part[(long)k * nn + t];
+    int k = 0;
+    for (; k + 8 <= nslice; k += 8) {
+        P v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = part[(long)(k + u) * nn + t];
+#pragma unroll
+        for (int u = 0; u < 8; u++) s += (double)v[u];
+    }
+    for (; k < nslice; k++) s += (double)part[(long)k * nn + t];
     out[t] = s;
 }
 
