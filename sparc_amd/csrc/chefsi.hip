@@ -1684,14 +1684,17 @@ int eigensolve_device(sgpu_context *ctx, int n, double *dA, double *dB, double *
     // once lambda is on the host)
     const size_t npub = 2 * (2 * (size_t)n + kCoopMaxG);
     const size_t nl11 = (size_t)((n + kCholNB - 1) / kCholNB) * kCholNB * kCholNB;
-    SGPU_CHECK(sgpu_scratch(ctx, (3 * (size_t)n + (size_t)n * n + npub + nl11) * sizeof(double) +
-                                     sizeof(CoopSync) + 64,
+    const bool tdc = ctx->own_stedc && n <= kTdcMaxN;
+    const size_t ntdc = tdc ? sgpu_tdc_work_doubles(n) : 0;
+    SGPU_CHECK(sgpu_scratch(ctx, (3 * (size_t)n + (size_t)n * n + npub + nl11 + ntdc) * sizeof(double) +
+                                     sizeof(CoopSync) + 320,
                             &buf));
     double *dD = (double *)buf, *dE = dD + n;
     double *pub = dE + 2 * n + (size_t)n * n;
     double *l11buf = pub + npub;
     CoopSync *sy = (CoopSync *)(l11buf + nl11);
     rocblas_int *dinfo = (rocblas_int *)(sy + 1);
+    double *tdcw = (double *)(((uintptr_t)(dinfo + 1) + 255) & ~(uintptr_t)255);
     sgpu_tmark t, tp;
     SGPU_CHECK(sgpu_tbegin(ctx, &t));
     SGPU_CHECK(sgpu_tbegin(ctx, &tp));
@@ -1762,7 +1765,11 @@ int eigensolve_device(sgpu_context *ctx, int n, double *dA, double *dB, double *
         }
         SGPU_CHECK(sgpu_tend(ctx, &tp, "eig_tridiag"));
         SGPU_CHECK(sgpu_tbegin(ctx, &tp));
-        SGPU_BLAS(rocsolver_dstedc(ctx->blas, rocblas_evect_tridiagonal, n, dD, dE, dC, n, dinfo));
+        if (tdc)
+            SGPU_CHECK(sgpu_tdc(ctx, n, dD, dE, dC, tdcw));
+        else
+            SGPU_BLAS(rocsolver_dstedc(ctx->blas, rocblas_evect_tridiagonal, n, dD, dE, dC, n,
+                                       dinfo));
         SGPU_CHECK(sgpu_tend(ctx, &tp, "eig_stedc"));
         SGPU_CHECK(sgpu_tbegin(ctx, &tp));
         SGPU_CHECK(sgpu_tbegin(ctx, &tq));

@@ -78,6 +78,7 @@ extern "C" int sgpu_create(sgpu_context **out, int device, int nx, int ny, int n
     if (const char *t = getenv("SPARC_AMD_COOP_THREADS")) ctx->coop_threads = atoi(t) == 512 ? 512 : 256;
     if (const char *t = getenv("SPARC_AMD_CHOL")) ctx->own_chol = strcmp(t, "rocsolver") != 0;
     if (const char *t = getenv("SPARC_AMD_ORMTR")) ctx->own_ormtr = strcmp(t, "rocsolver") != 0;
+    if (const char *t = getenv("SPARC_AMD_STEDC")) ctx->own_stedc = strcmp(t, "rocsolver") != 0;
     {
         int ncu = 0;
         if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&

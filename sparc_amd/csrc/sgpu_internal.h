@@ -153,6 +153,9 @@ struct sgpu_context {
     // the back-transformation Q1 Z by ormtr_lower_wave (SPARC_AMD_ORMTR=rocsolver
     // keeps rocSOLVER's dormtr)
     bool own_ormtr = true;
+    // the tridiagonal eigenvectors by sgpu_tdc's divide and conquer
+    // (SPARC_AMD_STEDC=rocsolver keeps rocSOLVER's dstedc)
+    bool own_stedc = true;
     int n_cu = 256;   // compute units of the device (set by sgpu_create)
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;
@@ -212,6 +215,14 @@ struct sgpu_tmark {
 int sgpu_tbegin(sgpu_context *ctx, sgpu_tmark *m);
 int sgpu_tend(sgpu_context *ctx, sgpu_tmark *m, const char *name);
 int sgpu_timer_resolve(sgpu_context *ctx);
+
+// ---- tridiagonal divide and conquer (tdc.hip) ----
+// Largest n sgpu_tdc takes, and its workspace in doubles for n.
+constexpr int kTdcMaxN = 1536;
+size_t sgpu_tdc_work_doubles(int n);
+// D (n) <- eigenvalues ascending, E (n - 1) the off-diagonal (read only),
+// Z (n x n, leading dimension n) <- the eigenvectors, on ctx->stream.
+int sgpu_tdc(sgpu_context *ctx, int n, double *dD, const double *dE, double *dZ, double *work);
 
 // ---- kernel launchers (stencil.hip) ----
 // out = s1 * ((a Lap + bdiag diag(v) + c) x) + s2 * z, per column (z may be null).

@@ -444,6 +444,51 @@ def test_generalized_eigensolve_large(gpu, n):
     assert np.max(np.abs(Q.T @ Hp @ Q - np.diag(lam))) < 1e-9
 
 
+@pytest.mark.parametrize("kind", ["random", "clusters", "graded", "multiple"])
+@pytest.mark.parametrize("n", [2, 24, 181, 480, 1000])
+def test_generalized_eigensolve_divide_and_conquer(gpu, monkeypatch, n, kind):
+    """The tridiagonal eigenproblem by the library's own divide and conquer
+    (1 x 1 leaves, dlaed2 deflation, the secular equation by the middle-way
+    iteration, Gu-Eisenstat vectors) against rocSOLVER's dstedc
+    (SPARC_AMD_STEDC=rocsolver) and LAPACK dsygvd: eigenvalues to 1e-10 of
+    the spectrum's scale, Mp-orthonormal eigenvectors that diagonalize Hp.
+    Spectra: random; sixfold clusters (C = V diag V' exactly degenerate,
+    deflation by rotation chains); graded over 8 decades; one eigenvalue of
+    multiplicity n (Hp = 2 Mp: every coupling deflates)."""
+    from sparc_amd.workloads import Grid
+    rng = np.random.default_rng(500 + n + len(kind))
+    d = 10.0 ** rng.uniform(-3, 3, n)
+    E = 1e-3 * rng.standard_normal((n, n))
+    Mp = np.sqrt(np.outer(d, d)) * (np.eye(n) + (E + E.T) / 2)
+    L = np.linalg.cholesky(Mp)
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    if kind == "random":
+        lam0 = rng.uniform(-0.3, 0.5, n)
+    elif kind == "clusters":
+        lam0 = np.repeat(rng.uniform(-0.3, 0.5, n // 6 + 1), 6)[:n]
+    elif kind == "graded":
+        lam0 = rng.choice([-1.0, 1.0], n) * 10.0 ** rng.uniform(-8, 0, n)
+    else:
+        lam0 = np.full(n, 2.0)
+    C = (V * lam0) @ V.T
+    Hp = L @ (0.5 * (C + C.T)) @ L.T
+    Hp = 0.5 * (Hp + Hp.T)
+    g = Grid(8, 8, 8, 0.5, 0.5, 0.5)
+    out = {}
+    for flag in ("own", "rocsolver"):
+        monkeypatch.setenv("SPARC_AMD_STEDC", flag)
+        with gpu.Context(g) as ctx:
+            out[flag] = ctx.DP_Solve_Generalized_EigenProblem(Hp, Mp)
+    lam, Q = out["own"]
+    scale = np.max(np.abs(lam0))
+    lam_ref = sla.eigh(Hp, Mp, eigvals_only=True, driver="gvd")
+    assert np.all(np.diff(lam) >= 0)
+    assert np.max(np.abs(lam - lam_ref)) < 1e-10 * scale
+    assert np.max(np.abs(lam - out["rocsolver"][0])) < 1e-10 * scale
+    assert np.max(np.abs(Q.T @ Mp @ Q - np.eye(n))) < 1e-9
+    assert np.max(np.abs(Q.T @ Hp @ Q - np.diag(lam))) < 1e-9 * scale
+
+
 @pytest.mark.parametrize("n", [480, 1100])
 def test_generalized_eigensolve_repeatable(gpu, n):
     """Four solves of the same pair in one context give bitwise the same
