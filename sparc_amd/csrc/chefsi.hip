@@ -186,6 +186,27 @@ __device__ __forceinline__ unsigned short bf16_bits(float f)
 }
 __device__ __forceinline__ float bf16_val(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
 
+// Two FP32 values to a BF16 pair (x in the low half), rounded to nearest
+// even: one v_cvt_pk_bf16_f32 on gfx950, the same bits as bf16_bits.
+typedef __bf16 bf16x2_pk_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_pk_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ unsigned bf16_pk(float x, float y)
+{
+    const f32x2_pk_t f = {x, y};
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(f, bf16x2_pk_t));
+}
+
+// split3 of two values at once, each piece as a packed pair (x low): three
+// conversions, four unpacks and four subtractions for the two.
+__device__ __forceinline__ void split3_pk(float x, float y, unsigned &h, unsigned &m, unsigned &l)
+{
+    h = bf16_pk(x, y);
+    const float rx = x - __uint_as_float(h << 16), ry = y - __uint_as_float(h & 0xffff0000u);
+    m = bf16_pk(rx, ry);
+    l = bf16_pk(rx - __uint_as_float(m << 16), ry - __uint_as_float(m & 0xffff0000u));
+}
+
+
 __global__ __launch_bounds__(256) void gram_bf16x3(const float *__restrict__ A, long lda,
                                                    const float *__restrict__ B, long ldb, int n,
                                                    long nd, long kslice, float *__restrict__ part)
@@ -217,15 +238,14 @@ __global__ __launch_bounds__(256) void gram_bf16x3(const float *__restrict__ A, 
                 if (j0 + c < n) vb = B[(long)(j0 + c) * ldb + k];
             }
             const int o = c * kGLD + kk;
-            unsigned short p0 = bf16_bits(va), p1 = bf16_bits(va - bf16_val(p0));
-            sa[0][o] = p0;
-            sa[1][o] = p1;
-            sa[2][o] = bf16_bits(va - bf16_val(p0) - bf16_val(p1));
-            p0 = bf16_bits(vb);
-            p1 = bf16_bits(vb - bf16_val(p0));
-            sb[0][o] = p0;
-            sb[1][o] = p1;
-            sb[2][o] = bf16_bits(vb - bf16_val(p0) - bf16_val(p1));
+            unsigned h, m, l;
+            split3_pk(va, vb, h, m, l);
+            sa[0][o] = (unsigned short)h;
+            sa[1][o] = (unsigned short)m;
+            sa[2][o] = (unsigned short)l;
+            sb[0][o] = (unsigned short)(h >> 16);
+            sb[1][o] = (unsigned short)(m >> 16);
+            sb[2][o] = (unsigned short)(l >> 16);
         }
         __syncthreads();
 #pragma unroll
@@ -362,15 +382,19 @@ __global__ __launch_bounds__(64 * kRotWaves) void rot_bf16x3(const float *__rest
                 bv[ct][q] = *reinterpret_cast<const bf16x8_t *>(qp + q * plane + (long)col * nk + k0 + 8 * g);
         }
         __syncthreads();  // the previous chunk's operands have been read
+        static_assert((PTS * kRotK / NT) % 2 == 0, "values split in pairs");
 #pragma unroll
-        for (int r = 0; r < PTS * kRotK / NT; r++) {
-            const int e = threadIdx.x + NT * r;
-            const int kk = e / PTS, pp = e % PTS;
-            unsigned short a0, a1, a2;
-            split3(v[r], a0, a1, a2);
-            sa[0][pp * kRotLd + kk] = a0;
-            sa[1][pp * kRotLd + kk] = a1;
-            sa[2][pp * kRotLd + kk] = a2;
+        for (int r = 0; r < PTS * kRotK / NT; r += 2) {
+            const int e0 = threadIdx.x + NT * r, e1 = e0 + NT;
+            const int o0 = (e0 % PTS) * kRotLd + e0 / PTS, o1 = (e1 % PTS) * kRotLd + e1 / PTS;
+            unsigned h, m, l;
+            split3_pk(v[r], v[r + 1], h, m, l);
+            sa[0][o0] = (unsigned short)h;
+            sa[1][o0] = (unsigned short)m;
+            sa[2][o0] = (unsigned short)l;
+            sa[0][o1] = (unsigned short)(h >> 16);
+            sa[1][o1] = (unsigned short)(m >> 16);
+            sa[2][o1] = (unsigned short)(l >> 16);
         }
         __syncthreads();
 #pragma unroll
@@ -494,18 +518,12 @@ __global__ __launch_bounds__(512) void gram3_bf16(const float *__restrict__ A, l
         }
     };
     auto put = [&](unsigned short *dst, int o, float4 v) {
-        unsigned short p[4][3];
-        split3(v.x, p[0][0], p[0][1], p[0][2]);
-        split3(v.y, p[1][0], p[1][1], p[1][2]);
-        split3(v.z, p[2][0], p[2][1], p[2][2]);
-        split3(v.w, p[3][0], p[3][1], p[3][2]);
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            const unsigned long long w = (unsigned long long)p[0][q] | ((unsigned long long)p[1][q] << 16) |
-                                         ((unsigned long long)p[2][q] << 32) |
-                                         ((unsigned long long)p[3][q] << 48);
-            *reinterpret_cast<unsigned long long *>(dst + q * kG3T * kG3Ld + o) = w;
-        }
+        unsigned h0, m0, l0, h1, m1, l1;
+        split3_pk(v.x, v.y, h0, m0, l0);
+        split3_pk(v.z, v.w, h1, m1, l1);
+        *reinterpret_cast<uint2 *>(dst + o) = make_uint2(h0, h1);
+        *reinterpret_cast<uint2 *>(dst + kG3T * kG3Ld + o) = make_uint2(m0, m1);
+        *reinterpret_cast<uint2 *>(dst + 2 * kG3T * kG3Ld + o) = make_uint2(l0, l1);
     };
     load(k0);
     for (long kb = k0; kb < k1; kb += kG3K) {
