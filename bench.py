@@ -467,9 +467,10 @@ def eigensolve_bench(n=480, reps=5):
     states on a matrix pair with a Chebyshev-filtered block's spread of scales
     (Mp diagonal over eight decades, nearly diagonal Hp), per call and per
     phase (HIP events): blocked Cholesky + L^-1 + products, tridiagonal
-    reduction (cooperative launch), dstedc, dormtr, back-transformation; and
-    the same with rocSOLVER's dsyevd for the eigenproblem and with rocSOLVER's
-    dpotrf + triangular solves, for comparison."""
+    reduction (cooperative launch), tridiagonal divide and conquer, dormtr,
+    back-transformation; and the same with rocSOLVER's dstedc, with
+    rocSOLVER's dsyevd for the eigenproblem and with rocSOLVER's dpotrf +
+    triangular solves, for comparison."""
     from sparc_amd import gpu
     from sparc_amd.workloads import Grid
     rng = np.random.default_rng(3)
@@ -482,9 +483,11 @@ def eigensolve_bench(n=480, reps=5):
     phases = ("eig_potrf", "eig_cholesky", "eig_tridiag", "eig_stedc", "eig_ormtr", "eig_back",
               "eigensolve")
     out = {"n": n, "reps": reps}
-    settings = {"default": {}, "dsyevd": {"SPARC_AMD_COOP_TRIDIAG": "0"},
+    settings = {"default": {}, "rocsolver_stedc": {"SPARC_AMD_STEDC": "rocsolver"},
+                "dsyevd": {"SPARC_AMD_COOP_TRIDIAG": "0"},
                 "rocsolver_cholesky": {"SPARC_AMD_CHOL": "rocsolver"}}
-    saved = {k: os.environ.get(k) for k in ("SPARC_AMD_COOP_TRIDIAG", "SPARC_AMD_CHOL")}
+    saved = {k: os.environ.get(k)
+             for k in ("SPARC_AMD_COOP_TRIDIAG", "SPARC_AMD_CHOL", "SPARC_AMD_STEDC")}
     try:
         for name, env in settings.items():
             for k in saved:

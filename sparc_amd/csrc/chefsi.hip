@@ -359,11 +359,13 @@ __global__ __launch_bounds__(64 * kRotWaves) void rot_bf16x3(const float *__rest
     for (int pt = 0; pt < NPT; pt++)
 #pragma unroll
         for (int ct = 0; ct < kRotMaxCT; ct++) acc[pt][ct] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < nk; k0 += kRotK) {
-        // stage Y[p0 .. p0+PTS-1][k0 .. k0+31]: consecutive threads on
-        // consecutive points (256-byte column segments), all loads first
-        constexpr int NT = 64 * kRotWaves;
-        float v[PTS * kRotK / NT];
+    // Y[p0 .. p0+PTS-1][k0 .. k0+31] into registers: consecutive threads on
+    // consecutive points (256-byte column segments), all loads first; the
+    // next chunk's loads are issued once this chunk is staged, so they are in
+    // flight during its MFMAs
+    constexpr int NT = 64 * kRotWaves;
+    float v[PTS * kRotK / NT];
+    auto load = [&](int k0) {
 #pragma unroll
         for (int r = 0; r < PTS * kRotK / NT; r++) {
             const int e = threadIdx.x + NT * r;
@@ -372,6 +374,9 @@ __global__ __launch_bounds__(64 * kRotWaves) void rot_bf16x3(const float *__rest
             v[r] = Y[ok ? (long)(k0 + kk) * ldy + p0 + pp : 0L];
             if (!ok) v[r] = 0.f;
         }
+    };
+    load(0);
+    for (int k0 = 0; k0 < nk; k0 += kRotK) {
         // this wave's Q pieces of the chunk, one 16-column tile per ct
         bf16x8_t bv[kRotMaxCT][3];
 #pragma unroll
@@ -397,6 +402,7 @@ __global__ __launch_bounds__(64 * kRotWaves) void rot_bf16x3(const float *__rest
             sa[2][o1] = (unsigned short)(l >> 16);
         }
         __syncthreads();
+        if (k0 + kRotK < nk) load(k0 + kRotK);
 #pragma unroll
         for (int pt = 0; pt < NPT; pt++) {
             bf16x8_t av[3];
