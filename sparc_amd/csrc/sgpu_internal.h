@@ -156,6 +156,9 @@ struct sgpu_context {
     // the tridiagonal eigenvectors by sgpu_tdc's divide and conquer
     // (SPARC_AMD_STEDC=rocsolver keeps rocSOLVER's dstedc)
     bool own_stedc = true;
+    // its lowest levels (merges of up to this many poles) in one launch, one
+    // wave per leaf block (SPARC_AMD_TDC_LEAF=1|8|16|32; 1: none)
+    int tdc_leaf = 8;
     int n_cu = 256;   // compute units of the device (set by sgpu_create)
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;

@@ -724,6 +724,269 @@ __global__ __launch_bounds__(256) void tdc_gemm(int n, int s, const int *__restr
     }
 }
 
+// ---- the lowest levels in one launch ----
+// Merges of up to S poles run inside one wave per leaf block [a0, a0 + L)
+// (L = S but for the last block): every level of the block in LDS, the same
+// steps as the kernels above with lanes for elements: each merge's first
+// lane runs dlaed2's deflation scan sequentially (at most S poles) and its
+// rotations, one lane per root runs the middle-way iteration with the sums
+// taken in order, one lane per pole Gu and Eisenstat's z, one lane per root
+// its vector, and the lanes share the products.  The global levels then
+// start at merges of 2 S.  With one wave per CU every step here is latency:
+// S = 8 (three levels in one launch instead of fifteen) is the fastest on
+// MI355X (n = 480: 0.39 ms against 0.47 without and 0.50 with S = 32).
+constexpr int kLeafMax = 32;
+
+// Root i of the merge's secular equation (poles dd[0..K), z values zz), as
+// tdc_secular with sequential sums.
+__device__ void tdc_root_seq(const double *dd, const double *zz, int K, int i, double rho,
+                             double &o_out, double &t_out)
+{
+    const double rinv = 1.0 / rho;
+    const bool last = i == K - 1;
+    double o = dd[i], lo = 0.0, hi, tt;
+    double psi = 0.0, phi = 0.0, dpsi = 0.0, dphi = 0.0;
+    auto eval = [&](double x) {
+        psi = phi = dpsi = dphi = 0.0;
+        for (int j = 0; j < K; j++) {
+            const double r = 1.0 / ((dd[j] - o) - x);
+            const double tw = zz[j] * zz[j] * r;
+            if (j <= i) { psi += tw; dpsi += tw * r; }
+            else { phi += tw; dphi += tw * r; }
+        }
+        return rinv + psi + phi;
+    };
+    double g;
+    if (!last) {
+        const double mid = 0.5 * (dd[i + 1] - dd[i]);
+        g = eval(mid);
+        if (g > 0.0) {
+            hi = tt = mid;
+        } else {
+            o = dd[i + 1];
+            lo = tt = -mid;
+            hi = 0.0;
+            g = eval(tt);
+        }
+    } else {
+        double ws = 0.0;
+        for (int j = 0; j < K; j++) ws += zz[j] * zz[j];
+        hi = tt = rho * ws;
+        g = eval(tt);
+    }
+    const double di = dd[i] - o, di1 = last ? 0.0 : dd[i + 1] - o;
+    for (int it = 0; it < 160; it++) {
+        const double err = 8.0 * (phi - psi + rinv) + fabs(tt) * (dpsi + dphi);
+        if (fabs(g) <= kEps * err) break;
+        if (g > 0.0) hi = tt;
+        else lo = tt;
+        if (!(hi > lo) || hi - lo <= 2.0 * kEps * fmax(fabs(lo), fabs(hi))) break;
+        const double a1 = di - tt;
+        const double B = dpsi * a1 * a1;
+        double tn = 0.5 * (lo + hi);
+        if (!last) {
+            const double a2 = di1 - tt;
+            const double D = dphi * a2 * a2;
+            const double c0 = rinv + (psi - B / a1) + (phi - D / a2);
+            const double bq = c0 * (a1 + a2) + B + D;
+            const double r0 = a1 * a2 * g;
+            const double disc = bq * bq - 4.0 * c0 * r0;
+            if (disc >= 0.0) {
+                const double qq = 0.5 * (bq + copysign(sqrt(disc), bq));
+                const double e1 = qq != 0.0 ? r0 / qq : 0.0;
+                const double e2 = c0 != 0.0 ? qq / c0 : 0.0;
+                if (qq != 0.0 && lo < tt + e1 && tt + e1 < hi) tn = tt + e1;
+                else if (c0 != 0.0 && lo < tt + e2 && tt + e2 < hi) tn = tt + e2;
+            }
+        } else {
+            const double c0 = rinv + (psi - B / a1);
+            if (c0 > 0.0) {
+                const double x = tt + a1 + B / c0;
+                if (lo < x && x < hi) tn = x;
+            }
+        }
+        if (!(lo < tn && tn < hi)) tn = 0.5 * (lo + hi);
+        if (tn == tt) break;
+        tt = tn;
+        g = eval(tt);
+    }
+    o_out = o;
+    t_out = tt;
+}
+
+template <int S>
+__global__ __launch_bounds__(64) void tdc_leaf(const double *__restrict__ d,
+                                               const double *__restrict__ e, int n,
+                                               double *__restrict__ lam_out,
+                                               double *__restrict__ Zg, int ld)
+{
+    static_assert(S <= kLeafMax && S <= 64, "one lane per element");
+    __shared__ double Zs[2][S * S];   // column-major, local rows and columns
+    __shared__ double Us[S * S];
+    __shared__ double lam[2][S];
+    __shared__ double ds[S], zs[S], dfv[S], ndp[S], ndz[S], org[S], tau[S], zh[S];
+    __shared__ int cols[S], ndc[S], dfc[S], posr[S], posd[S];
+    __shared__ int Km[S];
+    __shared__ double rhm[S];
+    const int t = threadIdx.x, a0 = blockIdx.x * S, L = min(S, n - a0);
+    if (t < L) {
+        const int g = a0 + t;
+        double v = d[g];
+        if (g > 0) v -= fabs(e[g - 1]);
+        if (g + 1 < n) v -= fabs(e[g]);
+        lam[0][t] = v;
+    }
+    for (int f = t; f < S * S; f += 64) {
+        Zs[0][f] = (f / S == f % S) ? 1.0 : 0.0;
+        Zs[1][f] = 0.0;
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int s = 1; s < L; s *= 2) {
+        const double *Zc = Zs[cur];
+        double *Zn = Zs[cur ^ 1];
+        // merges [a, m) + [m, b) of the level; element t's merge
+        const int mg = t / (2 * s), a = 2 * s * mg;
+        const int m = min(a + s, L), b = min(a + 2 * s, L);
+        const bool partner = t < L && m < b;
+        // the two sorted runs merged by rank, z from the old eigenvectors
+        if (partner) {
+            const double sg = e[a0 + m - 1] >= 0.0 ? 1.0 : -1.0;
+            const double v = lam[cur][t];
+            int pos;
+            if (t < m) {
+                int lo = m, hi = b;
+                while (lo < hi) { const int md = (lo + hi) >> 1; if (lam[cur][md] < v) lo = md + 1; else hi = md; }
+                pos = (t - a) + (lo - m);
+            } else {
+                int lo = a, hi = m;
+                while (lo < hi) { const int md = (lo + hi) >> 1; if (lam[cur][md] <= v) lo = md + 1; else hi = md; }
+                pos = (t - m) + (lo - a);
+            }
+            ds[a + pos] = v;
+            zs[a + pos] = t < m ? Zc[t * S + (m - 1)] : sg * Zc[t * S + m];
+            cols[a + pos] = t;
+        }
+        __syncthreads();
+        // deflation (dlaed2's scan) and rotations by each merge's first lane
+        if (t < L && t == a) {
+            const int nl = b - a;
+            if (m >= b) {
+                for (int q = 0; q < nl; q++) { dfv[a + q] = lam[cur][a + q]; dfc[a + q] = a + q; }
+                Km[mg] = 0;
+                rhm[mg] = 0.0;
+            } else {
+                const double beta = e[a0 + m - 1];
+                double zz = 0.0, dm = 0.0, zm = 0.0;
+                for (int j = a; j < b; j++) { zz += zs[j] * zs[j]; dm = fmax(dm, fabs(ds[j])); }
+                const double nz = sqrt(zz), rho = fabs(beta) * zz;
+                for (int j = a; j < b; j++) { zs[j] = nz > 0.0 ? zs[j] / nz : 0.0; zm = fmax(zm, fabs(zs[j])); }
+                const double tol = 8.0 * kEps * fmax(dm, zm);
+                int K = 0, nd = 0;
+                // deflated values kept sorted (insertion, ties in scan order)
+                auto push_defl = [&](double v, int c) {
+                    int q = nd++;
+                    while (q > 0 && dfv[a + q - 1] > v) {
+                        dfv[a + q] = dfv[a + q - 1];
+                        dfc[a + q] = dfc[a + q - 1];
+                        q--;
+                    }
+                    dfv[a + q] = v;
+                    dfc[a + q] = c;
+                };
+                if (rho * zm <= tol) {
+                    for (int j = a; j < b; j++) push_defl(ds[j], cols[j]);
+                } else {
+                    int pj = -1;
+                    for (int j = a; j < b; j++) {
+                        if (rho * fabs(zs[j]) <= tol) { push_defl(ds[j], cols[j]); continue; }
+                        if (pj < 0) { pj = j; continue; }
+                        const double tau2 = hypot(zs[j], zs[pj]);
+                        const double C = zs[j] / tau2, Sn = -zs[pj] / tau2;
+                        if (fabs((ds[j] - ds[pj]) * C * Sn) <= tol) {
+                            const int cp = cols[pj], cn = cols[j];
+                            for (int r = a; r < b; r++) {
+                                const double x = Zc[cp * S + r], y = Zc[cn * S + r];
+                                Zs[cur][cp * S + r] = C * x + Sn * y;
+                                Zs[cur][cn * S + r] = C * y - Sn * x;
+                            }
+                            const double t2 = ds[pj] * C * C + ds[j] * Sn * Sn;
+                            ds[j] = ds[pj] * Sn * Sn + ds[j] * C * C;
+                            zs[j] = tau2;
+                            push_defl(t2, cp);
+                        } else {
+                            ndp[a + K] = ds[pj]; ndz[a + K] = zs[pj]; ndc[a + K] = cols[pj]; K++;
+                        }
+                        pj = j;
+                    }
+                    if (pj >= 0) { ndp[a + K] = ds[pj]; ndz[a + K] = zs[pj]; ndc[a + K] = cols[pj]; K++; }
+                }
+                Km[mg] = K;
+                rhm[mg] = rho;
+            }
+        }
+        __syncthreads();
+        const int K = t < L ? Km[mg] : 0, nd = t < L ? (b - a) - K : 0, i = t - a;
+        if (t < L && i < K) tdc_root_seq(ndp + a, ndz + a, K, i, rhm[mg], org[a + i], tau[a + i]);
+        __syncthreads();
+        // Gu-Eisenstat z, sorted positions, new eigenvalues
+        if (t < L && i < K) {
+            const double dj = ndp[a + i];
+            double pr = ((org[a + i] - dj) + tau[a + i]) / rhm[mg];
+            for (int q = 0; q < K; q++)
+                if (q != i) pr *= ((org[a + q] - dj) + tau[a + q]) / (ndp[a + q] - dj);
+            zh[a + i] = copysign(sqrt(fabs(pr)), ndz[a + i]);
+            const double v = org[a + i] + tau[a + i];
+            int lo = 0, hi = nd;
+            while (lo < hi) { const int md = (lo + hi) >> 1; if (dfv[a + md] < v) lo = md + 1; else hi = md; }
+            posr[a + i] = i + lo;
+            lam[cur ^ 1][a + i + lo] = v;
+        }
+        if (t < L && i < nd) {
+            const double v = dfv[a + i];
+            int lo = 0, hi = K;
+            while (lo < hi) { const int md = (lo + hi) >> 1; if (org[a + md] + tau[a + md] <= v) lo = md + 1; else hi = md; }
+            posd[a + i] = i + lo;
+            lam[cur ^ 1][a + i + lo] = v;
+        }
+        __syncthreads();
+        // the vector of root i: zhat / (d - lambda_i), normalised
+        if (t < L && i < K) {
+            double ss = 0.0;
+            for (int j = 0; j < K; j++) {
+                const double u = zh[a + j] / ((ndp[a + j] - org[a + i]) - tau[a + i]);
+                Us[(a + i) * S + a + j] = u;
+                ss += u * u;
+            }
+            const double rn = 1.0 / sqrt(ss);
+            for (int j = 0; j < K; j++) Us[(a + i) * S + a + j] *= rn;
+        }
+        __syncthreads();
+        // new eigenvectors of each merge's block at their sorted positions
+        for (int f = t; f < L * L; f += 64) {
+            const int r = f % L, c = f / L;
+            const int mc = c / (2 * s), ac = 2 * s * mc, bc = min(ac + 2 * s, L);
+            if (r < ac || r >= bc) continue;
+            const int Kc = Km[mc], ic = c - ac;
+            if (ic < Kc) {
+                double acc = 0.0;
+                for (int j = 0; j < Kc; j++) acc += Zc[ndc[ac + j] * S + r] * Us[(ac + ic) * S + ac + j];
+                Zn[(ac + posr[ac + ic]) * S + r] = acc;
+            } else {
+                const int q = ic - Kc;
+                Zn[(ac + posd[ac + q]) * S + r] = Zc[dfc[ac + q] * S + r];
+            }
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    if (t < L) lam_out[a0 + t] = lam[cur][t];
+    for (int f = t; f < L * L; f += 64) {
+        const int r = f % L, c = f / L;
+        Zg[(long)(a0 + c) * ld + a0 + r] = Zs[cur][c * S + r];
+    }
+}
+
 }  // namespace
 
 // Workspace doubles for n.
@@ -777,9 +1040,17 @@ int sgpu_tdc(sgpu_context *ctx, int n, double *dD, const double *dE, double *dZ,
     sgpu_tmark tm;
     SGPU_CHECK(sgpu_tbegin(ctx, &tm));
     tdc_init<<<(unsigned)((nn + 255) / 256), 256, 0, st>>>(dD, dE, n, la, Za, Zb, n);
+    const int leaf = ctx->tdc_leaf;   // 1: no leaf kernel
+    const unsigned nleaf = (unsigned)((n + leaf - 1) / leaf);
+    if (leaf == 8)
+        tdc_leaf<8><<<nleaf, 64, 0, st>>>(dD, dE, n, la, Za, n);
+    else if (leaf == 16)
+        tdc_leaf<16><<<nleaf, 64, 0, st>>>(dD, dE, n, la, Za, n);
+    else if (leaf == 32)
+        tdc_leaf<32><<<nleaf, 64, 0, st>>>(dD, dE, n, la, Za, n);
     SGPU_HIP(hipGetLastError());
     SGPU_CHECK(sgpu_tend(ctx, &tm, "tdc_init"));
-    for (int s = 1; s < n; s *= 2) {
+    for (int s = leaf; s < n; s *= 2) {
         const int nm = (n + 2 * s - 1) / (2 * s);
         const int nl = std::min(2 * s, n);
         const size_t shm = (size_t)nl * (6 * sizeof(double) + 10 * sizeof(int));

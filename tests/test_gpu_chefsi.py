@@ -489,6 +489,37 @@ def test_generalized_eigensolve_divide_and_conquer(gpu, monkeypatch, n, kind):
     assert np.max(np.abs(Q.T @ Hp @ Q - np.diag(lam))) < 1e-9 * scale
 
 
+@pytest.mark.parametrize("leaf", ["1", "16", "32"])
+def test_generalized_eigensolve_divide_and_conquer_leaf(gpu, monkeypatch, leaf):
+    """The lowest merge levels inside one wave per leaf block (default 8
+    poles; SPARC_AMD_TDC_LEAF=1 runs every level as separate launches, 16 and
+    32 larger leaves): the same eigenvalues as the default to 1e-12 and
+    Mp-orthonormal eigenvectors that diagonalize Hp, for a random and a
+    sixfold-clustered spectrum at n = 181 (partial last leaf)."""
+    from sparc_amd.workloads import Grid
+    n = 181
+    rng = np.random.default_rng(610)
+    d = 10.0 ** rng.uniform(-3, 3, n)
+    E = 1e-3 * rng.standard_normal((n, n))
+    Mp = np.sqrt(np.outer(d, d)) * (np.eye(n) + (E + E.T) / 2)
+    L = np.linalg.cholesky(Mp)
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    g = Grid(8, 8, 8, 0.5, 0.5, 0.5)
+    for lam0 in (rng.uniform(-0.3, 0.5, n), np.repeat(rng.uniform(-0.3, 0.5, n // 6 + 1), 6)[:n]):
+        C = (V * lam0) @ V.T
+        Hp = L @ (0.5 * (C + C.T)) @ L.T
+        Hp = 0.5 * (Hp + Hp.T)
+        out = {}
+        for flag in ("8", leaf):
+            monkeypatch.setenv("SPARC_AMD_TDC_LEAF", flag)
+            with gpu.Context(g) as ctx:
+                out[flag] = ctx.DP_Solve_Generalized_EigenProblem(Hp, Mp)
+        lam, Q = out[leaf]
+        assert np.max(np.abs(lam - out["8"][0])) < 1e-12
+        assert np.max(np.abs(Q.T @ Mp @ Q - np.eye(n))) < 1e-9
+        assert np.max(np.abs(Q.T @ Hp @ Q - np.diag(lam))) < 1e-9
+
+
 @pytest.mark.parametrize("n", [480, 1100])
 def test_generalized_eigensolve_repeatable(gpu, n):
     """Four solves of the same pair in one context give bitwise the same
