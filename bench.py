@@ -532,6 +532,9 @@ def main(argv=None):
                     help="profiling aid: run only the SCF, the Poisson comparison or the "
                          "480-state eigensolve phases and print its JSON object (not the "
                          "benchmark line)")
+    ap.add_argument("--eig-n", type=int, default=480,
+                    help="state count of --only eigensolve (up to 180 the reduction runs in one "
+                         "workgroup, without the cooperative launch)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one device copy of a 512-vector FP32 block (known bytes) so "
                          "that a profile holds a plain copy kernel to scale HBM counters by")
@@ -550,7 +553,7 @@ def main(argv=None):
         print(json.dumps(poisson_bench(*train_unet())))
         return 0
     if args.only == "eigensolve":
-        print(json.dumps(eigensolve_bench()))
+        print(json.dumps(eigensolve_bench(n=args.eig_n)))
         return 0
 
     from sparc_amd import gpu
