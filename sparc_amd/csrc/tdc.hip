@@ -8,15 +8,20 @@
 // with coupling beta solves diag(lambda) + rho z z' with z the last row of
 // the left block's eigenvectors and sign(beta) times the first row of the
 // right block's, rho = |beta| |z|^2, z normalised (LAPACK dlaed2-dlaed4):
+//  * tdc_leaf (one wave per block of 8 poles): the three lowest levels in
+//    one launch, every step below done by lanes in LDS;
 //  * tdc_prep (one block per merge): the two sorted runs merged by rank,
 //    deflation in sorted order (rho |z_j| <= tol: eigenvalue d_j, vector
 //    unchanged; two poles whose Givens rotation t c s <= tol: rotated,
-//    the first deflated), tol = 8 eps max(|d|max, |z|max); the deflated
-//    values sorted;
+//    the first deflated), tol = 8 eps max(|d|max, |z|max), evaluated in
+//    parallel with the sequential scan's decisions; the deflated values
+//    sorted; the kept columns grouped by the row halves they span;
 //  * tdc_secular (one wave per root): the root in (d_i, d_i+1) (the last in
-//    (d_K-1, d_K-1 + rho z'z]) by bisection on tau = lambda - o, o the pole
-//    nearer the root, so that d_j - lambda = (d_j - o) - tau keeps its
-//    relative accuracy;
+//    (d_K-1, d_K-1 + rho z'z]) as tau = lambda - o, o the pole nearer the
+//    root, so that d_j - lambda = (d_j - o) - tau keeps its relative
+//    accuracy, by Li's middle-way iteration (LAPACK dlaed4) inside a
+//    bracket; the waves without a root apply the deflation's rotation
+//    chains to the old eigenvectors;
 //  * tdc_zhat (one wave per pole): Gu and Eisenstat's z from the computed
 //    roots (Loewner's formula as a product of ratios), so that the vectors
 //    are orthogonal to working accuracy, and the sorted positions of the
@@ -88,7 +93,8 @@ __device__ __forceinline__ double tdc_wave_prod(double v)
 
 // Leaves: lambda_i = d_i - |e_i-1| - |e_i|, Z = I, Z2 = 0 (a level writes
 // only the diagonal blocks of its output, and these grow from level to
-// level, so entries outside them stay zero in both buffers).
+// level, so entries outside them stay zero in both buffers; tdc_leaf then
+// rewrites lambda and Z's diagonal blocks of its leaf size).
 __global__ void tdc_init(const double *__restrict__ d, const double *__restrict__ e, int n,
                          double *__restrict__ lam, double *__restrict__ Z,
                          double *__restrict__ Z2, int ld)
@@ -361,7 +367,8 @@ __global__ __launch_bounds__(256) void tdc_prep(const double *__restrict__ lam,
     }
 }
 
-// Merge and root index of a wave: merges of 2 s values, one wave per value.
+// Merge and index of a wave (merges of 2 s values, one wave per value);
+// true when the wave has a root (index below the merge's K).
 __device__ __forceinline__ bool tdc_wave_item(int s, int n, const int *Km, int &mg, int &i, int &a,
                                               int &K)
 {
