@@ -489,6 +489,63 @@ def test_generalized_eigensolve_divide_and_conquer(gpu, monkeypatch, n, kind):
     assert np.max(np.abs(Q.T @ Hp @ Q - np.diag(lam))) < 1e-9 * scale
 
 
+def _tridiagonal_case(kind, n, rng):
+    """Symmetric tridiagonal test matrices whose spectra the mathematics fixes
+    (Clement, 1-D Laplacian, diagonal) or that are hard for divide and
+    conquer (Wilkinson: pairs of eigenvalues equal to many digits; glued
+    Wilkinson: 21-point Wilkinson blocks coupled by 1e-12)."""
+    if kind == "clement":        # eigenvalues -(n-1), -(n-3), ..., n-3, n-1
+        d = np.zeros(n)
+        e = np.sqrt(np.arange(1, n) * (n - np.arange(1, n)))
+        exact = np.arange(-(n - 1), n, 2, dtype=float)
+    elif kind == "laplacian":    # eigenvalues 2 - 2 cos(k pi / (n + 1))
+        d = np.full(n, 2.0)
+        e = np.full(n - 1, -1.0)
+        exact = np.sort(2.0 - 2.0 * np.cos(np.arange(1, n + 1) * np.pi / (n + 1)))
+    elif kind == "diagonal":     # every coupling zero: the sorted diagonal
+        d = rng.uniform(-1.0, 1.0, n)
+        e = np.zeros(n - 1)
+        exact = np.sort(d)
+    elif kind == "wilkinson":
+        m = (n - 1) // 2
+        d = np.abs(np.arange(n) - m).astype(float)
+        e = np.ones(n - 1)
+        exact = None
+    else:                        # glued Wilkinson W21 blocks
+        w = np.abs(np.arange(21) - 10).astype(float)
+        nb = n // 21
+        d = np.concatenate([np.tile(w, nb), np.zeros(n - 21 * nb)])
+        e = np.ones(n - 1)
+        e[20:21 * nb - 1:21] = 1e-12
+        exact = None
+    return d, e, exact
+
+
+@pytest.mark.parametrize("kind", ["clement", "laplacian", "diagonal", "wilkinson", "glued"])
+@pytest.mark.parametrize("n", [150, 480])
+def test_generalized_eigensolve_tridiagonal_matrices(gpu, kind, n):
+    """With Mp = I and a tridiagonal Hp the divide and conquer receives that
+    tridiagonal matrix (the reduction leaves it as it is, up to signs):
+    eigenvalues against the closed forms (Clement's integers, the Laplacian's
+    cosines, the sorted diagonal) or LAPACK via scipy (Wilkinson and glued
+    Wilkinson) to 1e-10 of the spectrum's scale, orthonormal eigenvectors
+    that diagonalize the matrix; n = 150 takes the one-workgroup reduction,
+    480 the cooperative one."""
+    from sparc_amd.workloads import Grid
+    rng = np.random.default_rng(700 + n)
+    d, e, exact = _tridiagonal_case(kind, n, rng)
+    T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+    if exact is None:
+        exact = sla.eigh_tridiagonal(d, e, eigvals_only=True)
+    with gpu.Context(Grid(8, 8, 8, 0.5, 0.5, 0.5)) as ctx:
+        lam, Q = ctx.DP_Solve_Generalized_EigenProblem(T, np.eye(n))
+    scale = np.max(np.abs(exact))
+    assert np.all(np.diff(lam) >= 0)
+    assert np.max(np.abs(lam - exact)) < 1e-10 * scale
+    assert np.max(np.abs(Q.T @ Q - np.eye(n))) < 1e-9
+    assert np.max(np.abs(Q.T @ T @ Q - np.diag(lam))) < 1e-9 * scale
+
+
 @pytest.mark.parametrize("leaf", ["1", "16", "32"])
 def test_generalized_eigensolve_divide_and_conquer_leaf(gpu, monkeypatch, leaf):
     """The lowest merge levels inside one wave per leaf block (default 8
