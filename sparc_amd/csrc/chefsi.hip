@@ -590,6 +590,148 @@ __global__ __launch_bounds__(512) void gram3_bf16(const float *__restrict__ A, l
             }
 }
 
+// Mp = Y'Y and Hp = Y'(HY) in one launch (both symmetric: the same upper
+// tiles): gram3_bf16<true>'s tiles, waves and products, with the block's
+// column tile of Y split and staged once for both and the two B tiles (Y
+// and HY) beside it.  Three staged tiles need 32-point chunks to fit the LDS
+// (92 KB); a chunk is one MFMA K-step for each of the two products, so a
+// barrier pair still covers 96 products per wave.  The K-steps, slices and
+// the order of the six products per step are those of gram3_bf16 with the
+// same slice length, so each of Mp and Hp is bitwise what the single-product
+// kernel gives.
+constexpr int kG3DK = 32;           // K per chunk
+constexpr int kG3DLd = kG3DK + 8;   // LDS row (bf16): 80 bytes, conflict-free 16-byte reads
+constexpr size_t kG3DShm = 3 * 3 * (size_t)kG3T * kG3DLd * sizeof(unsigned short);
+
+__global__ __launch_bounds__(512) void gram3x2_bf16(const float *__restrict__ Y,
+                                                    const float *__restrict__ HY, long ld, int n,
+                                                    long nd, long kslice,
+                                                    float *__restrict__ partM,
+                                                    float *__restrict__ partH)
+{
+    extern __shared__ __align__(16) unsigned short g3dsm[];
+    constexpr int PL = kG3T * kG3DLd;   // one piece of one tile
+    unsigned short *sa = g3dsm, *sb = g3dsm + 3 * PL, *sc = g3dsm + 6 * PL;
+    const int tiles = (n + kG3T - 1) / kG3T;
+    const int lid = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    int t = lid % gridDim.x, ti = 0;
+    while (t >= tiles - ti) {
+        t -= tiles - ti;
+        ti++;
+    }
+    const int tj = ti + t;
+    const int slice = lid / gridDim.x;
+    const long k0 = (long)slice * kslice, k1 = min(nd, k0 + kslice);
+    const int i0 = ti * kG3T, j0 = tj * kG3T;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, g = lane >> 4;
+    const int wi = (wave >> 1) * 32, wj = (wave & 1) * 64;
+    f32x4_t am[2][4], ah[2][4];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            am[a][b] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+            ah[a][b] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+        }
+    constexpr int PER = kG3T * kG3DK / 4 / 512;
+    float4 va[PER], vb[PER], vc[PER];
+    auto load = [&](long kb) {
+#pragma unroll
+        for (int r = 0; r < PER; r++) {
+            const int e = threadIdx.x + 512 * r;
+            const int kq = 4 * (e % (kG3DK / 4)), c = e / (kG3DK / 4);
+            const long k = kb + kq;
+            const bool oka = k < k1 && i0 + c < n, okb = k < k1 && j0 + c < n;
+            va[r] = *reinterpret_cast<const float4 *>(Y + (oka ? (long)(i0 + c) * ld + k : 0L));
+            vb[r] = *reinterpret_cast<const float4 *>(Y + (okb ? (long)(j0 + c) * ld + k : 0L));
+            vc[r] = *reinterpret_cast<const float4 *>(HY + (okb ? (long)(j0 + c) * ld + k : 0L));
+            const int nv = (int)min(4L, k1 - k);
+            if (!oka) va[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (!okb) { vb[r] = make_float4(0.f, 0.f, 0.f, 0.f); vc[r] = vb[r]; }
+            if (nv < 4) { va[r].w = 0.f; vb[r].w = 0.f; vc[r].w = 0.f; }
+            if (nv < 3) { va[r].z = 0.f; vb[r].z = 0.f; vc[r].z = 0.f; }
+            if (nv < 2) { va[r].y = 0.f; vb[r].y = 0.f; vc[r].y = 0.f; }
+        }
+    };
+    auto put = [&](unsigned short *dst, int o, float4 v) {
+        unsigned h0, m0, l0, h1, m1, l1;
+        split3_pk(v.x, v.y, h0, m0, l0);
+        split3_pk(v.z, v.w, h1, m1, l1);
+        *reinterpret_cast<uint2 *>(dst + o) = make_uint2(h0, h1);
+        *reinterpret_cast<uint2 *>(dst + PL + o) = make_uint2(m0, m1);
+        *reinterpret_cast<uint2 *>(dst + 2 * PL + o) = make_uint2(l0, l1);
+    };
+    load(k0);
+    for (long kb = k0; kb < k1; kb += kG3DK) {
+        __syncthreads();  // the previous chunk's operands have been read
+#pragma unroll
+        for (int r = 0; r < PER; r++) {
+            const int e = threadIdx.x + 512 * r;
+            const int kq = 4 * (e % (kG3DK / 4)), c = e / (kG3DK / 4);
+            const int o = c * kG3DLd + kq;
+            put(sa, o, va[r]);
+            put(sb, o, vb[r]);
+            put(sc, o, vc[r]);
+        }
+        __syncthreads();
+        if (kb + kG3DK < k1) load(kb + kG3DK);
+        bf16x8_t av[2][3];
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int q = 0; q < 3; q++)
+                av[a][q] = *reinterpret_cast<const bf16x8_t *>(
+                    &sa[q * PL + (wi + a * 16 + li) * kG3DLd + 8 * g]);
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            bf16x8_t bv[3], cv[3];
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                bv[q] = *reinterpret_cast<const bf16x8_t *>(&sb[q * PL + (wj + b * 16 + li) * kG3DLd + 8 * g]);
+                cv[q] = *reinterpret_cast<const bf16x8_t *>(&sc[q * PL + (wj + b * 16 + li) * kG3DLd + 8 * g]);
+            }
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                f32x4_t c = am[a][b];
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][1], bv[1], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][0], bv[2], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][2], bv[0], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][0], bv[1], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][1], bv[0], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][0], bv[0], c, 0, 0, 0);
+                am[a][b] = c;
+                f32x4_t d = ah[a][b];
+                d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][1], cv[1], d, 0, 0, 0);
+                d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][0], cv[2], d, 0, 0, 0);
+                d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][2], cv[0], d, 0, 0, 0);
+                d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][0], cv[1], d, 0, 0, 0);
+                d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][1], cv[0], d, 0, 0, 0);
+                d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][0], cv[0], d, 0, 0, 0);
+                ah[a][b] = d;
+            }
+        }
+    }
+    const long off = (long)slice * n * n;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int i = i0 + wi + a * 16 + 4 * g + r;
+                const int j = j0 + wj + b * 16 + li;
+                if (i < n && j < n) {
+                    partM[off + (long)j * n + i] = am[a][b][r];
+                    partH[off + (long)j * n + i] = ah[a][b][r];
+                    if (ti != tj) {
+                        partM[off + (long)i * n + j] = am[a][b][r];
+                        partH[off + (long)i * n + j] = ah[a][b][r];
+                    }
+                }
+            }
+}
+
 template <typename T>
 rocblas_status gemm_tn_batched(rocblas_handle h, int n, long k, const T *A, long lda, long sa,
                                const T *B, long ldb, long sb, T *C, long sc, int batch);
@@ -740,6 +882,43 @@ int gram_tn(sgpu_context *ctx, int dtype, int n, long nd, const void *A, long ld
                                 symmetric);
 }
 
+// Mp = Y'Y and Hp = Y'HY by gram3x2_bf16 (FP32 from 256 states, the
+// 128 x 128 BF16-piece kernel's conditions); false when it does not apply
+// (SPARC_AMD_GRAM3_PAIR=0: two gram3_bf16 launches instead).
+static bool gram3_pair_ok(const sgpu_context *ctx, int dtype, int n, const void *Y,
+                          const void *HY, long ld)
+{
+    static const bool on = [] {
+        const char *e = getenv("SPARC_AMD_GRAM3_PAIR");
+        return !(e && atoi(e) == 0);
+    }();
+    (void)ctx;
+    return on && dtype == SGPU_F32 && n >= 256 && gram_mode() == 0 && ld % 4 == 0 &&
+           (reinterpret_cast<uintptr_t>(Y) & 15) == 0 && (reinterpret_cast<uintptr_t>(HY) & 15) == 0;
+}
+
+static int gram3_pair(sgpu_context *ctx, int n, long nd, const float *Y, const float *HY, long ld,
+                      double *Mp, double *Hp)
+{
+    const int tiles = (n + kG3T - 1) / kG3T;
+    const unsigned nblk = (unsigned)(tiles * (tiles + 1) / 2);
+    const long kslice = gram3_kslice(ctx, nblk, nd);   // the single-product kernel's slices
+    const int nslice = (int)((nd + kslice - 1) / kslice);
+    const long nn = (long)n * n;
+    void *buf = nullptr;
+    SGPU_CHECK(sgpu_scratch(ctx, 2 * (size_t)nslice * nn * sizeof(float), &buf));
+    float *pm = (float *)buf, *ph = pm + (size_t)nslice * nn;
+    SGPU_HIP(hipFuncSetAttribute((const void *)gram3x2_bf16,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)kG3DShm));
+    gram3x2_bf16<<<dim3(nblk, nslice), 512, kG3DShm, ctx->stream>>>(Y, HY, ld, n, nd, kslice, pm,
+                                                                   ph);
+    SGPU_HIP(hipGetLastError());
+    sum_partials<float><<<(unsigned)((nn + 255) / 256), 256, 0, ctx->stream>>>(pm, nslice, nn, Mp);
+    sum_partials<float><<<(unsigned)((nn + 255) / 256), 256, 0, ctx->stream>>>(ph, nslice, nn, Hp);
+    SGPU_HIP(hipGetLastError());
+    return SGPU_OK;
+}
+
 // HY = H Y, then Mp = Y'Y at *dmat and Hp = Y'HY at *dmat + ncol^2 (FP64,
 // device, in the context's reduction buffer).
 int project_device(sgpu_context *ctx, int dtype, int ncol, const void *Y, void *HY, long ld,
@@ -751,8 +930,13 @@ int project_device(sgpu_context *ctx, int dtype, int ncol, const void *Y, void *
     const long nn = (long)ncol * ncol;
     SGPU_CHECK(sgpu_reduce_buf(ctx, (size_t)(2 * nn), &dmat));
     SGPU_CHECK(sgpu_tbegin(ctx, &t));
-    SGPU_CHECK(gram_tn(ctx, dtype, ncol, ctx->nd, Y, ld, Y, ld, dmat, true));
-    SGPU_CHECK(gram_tn(ctx, dtype, ncol, ctx->nd, Y, ld, HY, ld, dmat + nn, true));
+    if (gram3_pair_ok(ctx, dtype, ncol, Y, HY, ld)) {
+        SGPU_CHECK(gram3_pair(ctx, ncol, ctx->nd, (const float *)Y, (const float *)HY, ld, dmat,
+                              dmat + nn));
+    } else {
+        SGPU_CHECK(gram_tn(ctx, dtype, ncol, ctx->nd, Y, ld, Y, ld, dmat, true));
+        SGPU_CHECK(gram_tn(ctx, dtype, ncol, ctx->nd, Y, ld, HY, ld, dmat + nn, true));
+    }
     SGPU_CHECK(sgpu_tend(ctx, &t, "project_gemm"));
     *dmat_out = dmat;
     return SGPU_OK;
