@@ -888,12 +888,7 @@ int gram_tn(sgpu_context *ctx, int dtype, int n, long nd, const void *A, long ld
 static bool gram3_pair_ok(const sgpu_context *ctx, int dtype, int n, const void *Y,
                           const void *HY, long ld)
 {
-    static const bool on = [] {
-        const char *e = getenv("SPARC_AMD_GRAM3_PAIR");
-        return !(e && atoi(e) == 0);
-    }();
-    (void)ctx;
-    return on && dtype == SGPU_F32 && n >= 256 && gram_mode() == 0 && ld % 4 == 0 &&
+    return ctx->gram3_pair && dtype == SGPU_F32 && n >= 256 && gram_mode() == 0 && ld % 4 == 0 &&
            (reinterpret_cast<uintptr_t>(Y) & 15) == 0 && (reinterpret_cast<uintptr_t>(HY) & 15) == 0;
 }
 

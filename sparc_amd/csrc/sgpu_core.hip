@@ -79,6 +79,7 @@ extern "C" int sgpu_create(sgpu_context **out, int device, int nx, int ny, int n
     if (const char *t = getenv("SPARC_AMD_CHOL")) ctx->own_chol = strcmp(t, "rocsolver") != 0;
     if (const char *t = getenv("SPARC_AMD_ORMTR")) ctx->own_ormtr = strcmp(t, "rocsolver") != 0;
     if (const char *t = getenv("SPARC_AMD_STEDC")) ctx->own_stedc = strcmp(t, "rocsolver") != 0;
+    if (const char *t = getenv("SPARC_AMD_GRAM3_PAIR")) ctx->gram3_pair = atoi(t) != 0;
     if (const char *t = getenv("SPARC_AMD_TDC_LEAF")) {
         const int k = atoi(t);
         if (k == 1 || k == 8 || k == 16 || k == 32) ctx->tdc_leaf = k;

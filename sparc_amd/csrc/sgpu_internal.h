@@ -159,6 +159,9 @@ struct sgpu_context {
     // its lowest levels (merges of up to this many poles) in one launch, one
     // wave per leaf block (SPARC_AMD_TDC_LEAF=1|8|16|32; 1: none)
     int tdc_leaf = 8;
+    // Mp and Hp of a CheFSI projection in one BF16-piece launch
+    // (SPARC_AMD_GRAM3_PAIR=0: two)
+    bool gram3_pair = true;
     int n_cu = 256;   // compute units of the device (set by sgpu_create)
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;

@@ -489,6 +489,30 @@ def test_generalized_eigensolve_divide_and_conquer(gpu, monkeypatch, n, kind):
     assert np.max(np.abs(Q.T @ Hp @ Q - np.diag(lam))) < 1e-9 * scale
 
 
+@pytest.mark.parametrize("ncol", [256, 300])
+def test_projection_pair_kernel_bitwise(gpu, monkeypatch, ncol):
+    """Mp = Y'Y and Hp = Y'HY of an FP32 block from the one-launch pair kernel
+    (default from 256 states) against the two single-product launches
+    (SPARC_AMD_GRAM3_PAIR=0): bitwise equal; 300 columns leave partial tiles."""
+    from sparc_amd.workloads import Grid
+    g = Grid(24, 24, 24, 0.5, 0.5, 0.5)
+    rng = np.random.default_rng(810 + ncol)
+    Y = rng.standard_normal((g.nd, ncol))
+    veff = rng.uniform(-1.0, 0.0, g.nd)
+    out = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("SPARC_AMD_GRAM3_PAIR", flag)
+        with gpu.Context(g) as ctx:
+            ctx.set_veff(veff)
+            by = ctx.block(ncol, gpu.F32).upload(Y)
+            bh = ctx.block(ncol, gpu.F32)
+            out[flag] = ctx.project_dev(by, bh)
+    assert np.array_equal(out["1"][0], out["0"][0]) and np.array_equal(out["1"][1], out["0"][1])
+    Yf = Y.astype(np.float32).astype(np.float64)
+    Mref = Yf.T @ Yf
+    assert np.max(np.abs(out["1"][1] - Mref)) < 1e-5 * np.max(np.abs(Mref))
+
+
 def _tridiagonal_case(kind, n, rng):
     """Symmetric tridiagonal test matrices whose spectra the mathematics fixes
     (Clement, 1-D Laplacian, diagonal) or that are hard for divide and
