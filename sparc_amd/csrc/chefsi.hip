@@ -882,13 +882,14 @@ int gram_tn(sgpu_context *ctx, int dtype, int n, long nd, const void *A, long ld
                                 symmetric);
 }
 
-// Mp = Y'Y and Hp = Y'HY by gram3x2_bf16 (FP32 from 256 states, the
-// 128 x 128 BF16-piece kernel's conditions); false when it does not apply
-// (SPARC_AMD_GRAM3_PAIR=0: two gram3_bf16 launches instead).
+// Mp = Y'Y and Hp = Y'HY by gram3x2_bf16 (FP32 from 128 states: at 160
+// states on the 64-atom cell 0.32 ms against 0.54 for the 64 x 64 kernel's
+// two launches); false when it does not apply (SPARC_AMD_GRAM3_PAIR=0: two
+// gram_tn launches instead).
 static bool gram3_pair_ok(const sgpu_context *ctx, int dtype, int n, const void *Y,
                           const void *HY, long ld)
 {
-    return ctx->gram3_pair && dtype == SGPU_F32 && n >= 256 && gram_mode() == 0 && ld % 4 == 0 &&
+    return ctx->gram3_pair && dtype == SGPU_F32 && n >= 128 && gram_mode() == 0 && ld % 4 == 0 &&
            (reinterpret_cast<uintptr_t>(Y) & 15) == 0 && (reinterpret_cast<uintptr_t>(HY) & 15) == 0;
 }
 
