@@ -323,8 +323,9 @@ def train_unet(train_steps=400):
     """Train the learned Poisson preconditioner on the GPU, on (residual,
     error) pairs harvested from Jacobi-AAR solves of the 8-atom (26^3, run on
     28^3 by interpolation) and 64-atom (52^3) Si SCFs' Poisson problems, Adam
-    in blocks alternating between the two grids, the weights with the fewest
-    AAR iterations on a held-out 64-atom problem kept (sparc_amd.learned).
+    in blocks alternating between the two grids, the earliest weights with the
+    fewest AAR iterations summed over held-out 8- and 64-atom problems kept
+    (sparc_amd.learned).
     Returns the FP32 weights and a record of the training."""
     from sparc_amd.learned import train_poisson_unet
     w, info = train_poisson_unet(("si8", "si64"), steps=train_steps)

@@ -9,10 +9,11 @@ each gives a (residual, error) pair (sgpu_unet_harvest).  Pairs of several
 grids train one set of weights: Adam runs in blocks of steps on each grid's
 network in turn, the weights carried from one to the next
 (gpu.train_multigrid).  After every block the current weights solve a
-held-out problem (the density after 3 SCF iterations of the last system) with
-AAR to the same tolerance, and the weights with the fewest AAR iterations are
-kept: a block after which the network stops preconditioning (which longer
-training can reach) is never the one returned.
+held-out problem per system (the density after 3 SCF iterations) with AAR to
+the same tolerance, and the earliest weights with the fewest AAR iterations
+summed over the systems are kept (early stopping): a block after which the
+network stops preconditioning (which longer training can reach) is never the
+one returned, and of equally good checkpoints the one trained least.
 """
 from __future__ import annotations
 
@@ -41,7 +42,7 @@ def train_poisson_unet(systems=("si8", "si64"), steps: int = 400, block: int = 5
     the training and the validation solves."""
     t0 = time.perf_counter()
     ctxs, nets, pairs = [], [], {}
-    val = None
+    vals = []
     try:
         for name in systems:
             st, p = scf_system(name)
@@ -55,23 +56,27 @@ def train_poisson_unet(systems=("si8", "si64"), steps: int = 400, block: int = 5
                 net.harvest(4.0 * np.pi * (st.b + rho), np.zeros(st.grid.nd), tol=1e-8,
                             max_iter=3000, stride=2, max_pairs=max_pairs)
             pairs[name] = net.npairs
-            if validate and name == systems[-1]:
+            if validate:
                 rho3 = scf_densities(ctx, st, p, (3,))[1]
-                val = (ctx, net, 4.0 * np.pi * (st.b + rho3), st.grid.nd)
+                vals.append((ctx, net, 4.0 * np.pi * (st.b + rho3), st.grid.nd))
         t_harvest = time.perf_counter() - t0
         for net in nets[1:]:
             net.set_weights(nets[0].get_weights())
 
         def val_iters(w):
-            ctx, net, rhs, nd = val
-            net.set_weights(w)
-            _, it, _, stt = ctx.AAR_unet(net, np.zeros(nd), rhs, tol=1e-8, max_iter=3000)
-            return it if stt == 0 else 10 ** 9
+            tot = 0
+            for ctx, net, rhs, nd in vals:
+                net.set_weights(w)
+                _, it, _, stt = ctx.AAR_unet(net, np.zeros(nd), rhs, tol=1e-8, max_iter=3000)
+                if stt != 0:
+                    return 10 ** 9
+                tot += it
+            return tot
 
         t1 = time.perf_counter()
         hist = []
         w = nets[0].get_weights()
-        best = (val_iters(w) if val else 0, 0, w)  # (iterations, steps, weights): U = 0 first
+        best = (val_iters(w) if vals else 0, 0, w)  # (iterations, steps, weights): U = 0 first
         val_log = [(0, best[0])]
         done = 0
         while done < steps:
@@ -80,10 +85,10 @@ def train_poisson_unet(systems=("si8", "si64"), steps: int = 400, block: int = 5
             h, w = gpu.train_multigrid(nets, n, batch=batch, lr=lr, seed=seed + done, block=block)
             hist.append(h)
             done += n
-            if val:
+            if vals:
                 it = val_iters(w)
                 val_log.append((done, it))
-                if it <= best[0]:
+                if it < best[0]:
                     best = (it, done, w)
             else:
                 best = (0, done, w)
@@ -97,7 +102,8 @@ def train_poisson_unet(systems=("si8", "si64"), steps: int = 400, block: int = 5
                 "validation_aar_iterations": val_log, "kept_after_steps": best[1],
                 "source": "Jacobi-AAR Poisson solves (tol 1e-8) of the SCF densities (guess and "
                           "after 1, 2, 4 iterations) of " + ", ".join(systems)
-                          + "; validation: the density after 3 iterations of " + systems[-1]}
+                          + "; validation: the densities after 3 iterations of " + ", ".join(systems)
+                          + " (AAR iterations summed, the earliest of the fewest kept)"}
         return best[2], info
     finally:
         for net in nets:

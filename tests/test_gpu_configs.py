@@ -192,8 +192,8 @@ def test_si64_mixed_scf_matches_fp64(si64):
 def unet_weights():
     """The network trained as the benchmark trains it: pairs of the 8-atom
     (26^3, run on 28^3 by interpolation) and 64-atom (52^3) cells' Poisson
-    solves, the weights with the fewest AAR iterations on a held-out 64-atom
-    problem kept."""
+    solves, the earliest weights with the fewest AAR iterations summed over
+    held-out 8- and 64-atom problems kept."""
     from sparc_amd.learned import train_poisson_unet
     _step("U-Net training on si8 + si64 pairs")
     w, info = train_poisson_unet(("si8", "si64"), steps=400)
