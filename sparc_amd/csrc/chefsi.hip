@@ -2016,6 +2016,11 @@ int eigensolve_device(sgpu_context *ctx, int n, double *dA, double *dB, double *
         sgpu_set_error("eigensolve: tridiagonal eigensolve did not converge (info = %d)", (int)info);
         return SGPU_ERR_SOLVER;
     }
+    for (int i = 0; i < n; i++)
+        if (!std::isfinite(lambda[i])) {
+            sgpu_set_error("eigensolve: non-finite eigenvalue %d (non-finite Hp or Mp?)", i);
+            return SGPU_ERR_SOLVER;
+        }
     return SGPU_OK;
 }
 
@@ -2369,6 +2374,15 @@ extern "C" int sgpu_DP_Solve_Generalized_EigenProblem(sgpu_context *ctx, int n, 
         sgpu_set_error("sgpu_DP_Solve_Generalized_EigenProblem: bad arguments");
         return SGPU_ERR_ARG;
     }
+    // LAPACK would report non-finite input through info; the device path has
+    // no such report, so it is refused here before anything is uploaded
+    const long nn = (long)n * n;
+    for (long i = 0; i < nn; i++)
+        if (!std::isfinite(Hp[i]) || !std::isfinite(Mp[i])) {
+            sgpu_set_error("sgpu_DP_Solve_Generalized_EigenProblem: non-finite entry in %s at %ld",
+                           std::isfinite(Hp[i]) ? "Mp" : "Hp", i);
+            return SGPU_ERR_ARG;
+        }
     SGPU_HIP(hipSetDevice(ctx->device));
     return eigensolve(ctx, n, Hp, Mp, lambda);
 }

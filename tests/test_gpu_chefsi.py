@@ -520,6 +520,26 @@ def test_projection_pair_kernel_bitwise(gpu, monkeypatch, ncol):
     assert np.max(np.abs(out["1"][1] - Mref)) < 1e-5 * np.max(np.abs(Mref))
 
 
+def test_generalized_eigensolve_refuses_non_finite(gpu):
+    """A NaN or infinity in Hp or Mp is refused on the host before any upload
+    (LAPACK's dsygvd would report it through info), with the matrix and the
+    entry named; the context stays usable."""
+    from sparc_amd.workloads import Grid
+    n = 32
+    rng = np.random.default_rng(900)
+    A = rng.standard_normal((n, n))
+    Hp = A + A.T
+    Mp = np.eye(n) + 0.01 * (A @ A.T)
+    with gpu.Context(Grid(8, 8, 8, 0.5, 0.5, 0.5)) as ctx:
+        for bad_h in (True, False):
+            H, M = Hp.copy(), Mp.copy()
+            (H if bad_h else M)[3, 5] = np.nan if bad_h else np.inf
+            with pytest.raises(gpu.SparcGPUError, match="non-finite entry in " + ("Hp" if bad_h else "Mp")):
+                ctx.DP_Solve_Generalized_EigenProblem(H, M)
+        lam, _ = ctx.DP_Solve_Generalized_EigenProblem(Hp, Mp)
+    assert np.max(np.abs(lam - sla.eigh(Hp, Mp, eigvals_only=True))) < 1e-10 * np.max(np.abs(lam))
+
+
 def _tridiagonal_case(kind, n, rng):
     """Symmetric tridiagonal test matrices whose spectra the mathematics fixes
     (Clement, 1-D Laplacian, diagonal) or that are hard for divide and
